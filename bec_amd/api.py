@@ -1,0 +1,353 @@
+"""ctypes binding of libbec_amd.so (the C ABI declared in include/bec_api.h).
+
+The product path always runs the HIP kernels: if the shared library is
+missing this module raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbec_amd.so")
+
+OK = 0
+ERR_INVALID, ERR_SHAPE, ERR_ALLOC, ERR_HIP, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_STATE, ERR_UNSUPPORTED = range(1, 9)
+FP32, FP64 = 0, 1
+SPACE_X, SPACE_K = 0, 1
+PULSE_HALF_PI, PULSE_PI = 0, 1
+FFT_FORWARD, FFT_INVERSE = -1, 1
+GS_SPLIT_STEP, GS_GRADIENT = 0, 1
+GS_RULE_REFERENCE, GS_RULE_RESIDUAL = 0, 1
+
+
+class BecError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        msg = lib().bec_strerror(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{where}: {msg} (status {status})")
+
+
+class ModelParams(C.Structure):
+    """ModelParameters (programs/bec_wigner_3d.cuda/defines.h:25-54)."""
+    _fields_ = [("a11", C.c_double), ("a12", C.c_double), ("a22", C.c_double), ("m", C.c_double),
+                ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double), ("N", C.c_int),
+                ("detuning", C.c_double), ("gamma111", C.c_double), ("gamma12", C.c_double),
+                ("gamma22", C.c_double), ("Va", C.c_double), ("Vb", C.c_double),
+                ("nvx", C.c_int), ("nvy", C.c_int), ("nvz", C.c_int), ("itmax", C.c_int),
+                ("dt_steady", C.c_double), ("t_equilib", C.c_double), ("dt_evo", C.c_double),
+                ("ne", C.c_int)]
+
+
+class CalcParams(C.Structure):
+    """CalculationParameters (programs/bec_wigner_3d.cuda/defines.h:57-112)."""
+    _fields_ = [("N", C.c_int), ("g11", C.c_double), ("g12", C.c_double), ("g22", C.c_double),
+                ("mu", C.c_double), ("cells", C.c_int), ("Va", C.c_double), ("Vb", C.c_double),
+                ("V", C.c_double), ("lambda_", C.c_double), ("l_rho", C.c_double), ("t_rho", C.c_double),
+                ("detuning", C.c_double), ("l111", C.c_double), ("l12", C.c_double), ("l22", C.c_double),
+                ("nvx", C.c_int), ("nvy", C.c_int), ("nvz", C.c_int),
+                ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("dkx", C.c_double), ("dky", C.c_double), ("dkz", C.c_double),
+                ("itmax", C.c_int), ("dt_steady", C.c_double), ("t_equilib", C.c_double),
+                ("dt_evo", C.c_double), ("ne", C.c_int)]
+
+
+class GsOptions(C.Structure):
+    _fields_ = [("method", C.c_int), ("rule", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
+                ("check_every", C.c_int), ("precision", C.c_int), ("alpha", C.c_double),
+                ("beta", C.c_double), ("tau", C.c_double)]
+
+
+class GsReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("E_ref", C.c_double),
+                ("mu_ref", C.c_double), ("E", C.c_double), ("mu", C.c_double), ("N", C.c_double),
+                ("residual", C.c_double), ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Obs(C.Structure):
+    _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
+                ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
+                ("N1", C.c_double), ("N2", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+EXPORTS = {
+    "bec_default_model_params": (C.c_int, [C.POINTER(ModelParams)]),
+    "bec_fill_derived_params": (C.c_int, [C.POINTER(ModelParams), C.POINTER(CalcParams)]),
+    "bec_create": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, C.POINTER(_P)]),
+    "bec_create_groups": (C.c_int, [C.POINTER(CalcParams), C.c_int, C.c_int, _P, C.POINTER(_P)]),
+    "bec_destroy": (C.c_int, [_P]),
+    "bec_strerror": (C.c_char_p, [C.c_int]),
+    "bec_synchronize": (C.c_int, [_P]),
+    "bec_group_count": (C.c_int, [_P]),
+    "bec_default_gs_options": (C.c_int, [C.POINTER(GsOptions)]),
+    "bec_calculate_steady_state": (C.c_int, [_P, C.c_int, _P, C.POINTER(GsOptions), _P, C.POINTER(GsReport)]),
+    "bec_ground_states_batched": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, C.POINTER(GsOptions), _P, _P,
+                                            C.POINTER(GsReport)]),
+    "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
+    "bec_init_evolution": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int]),
+    "bec_propagate": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _P]),
+    "bec_propagate_graph": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
+    "bec_observables": (C.c_int, [_P, C.c_int, C.POINTER(Obs)]),
+    "bec_calculate_evolution": (C.c_int, [_P, C.c_double, C.POINTER(Obs)]),
+    "bec_trajectory_sums": (C.c_int, [_P, C.c_int, _P]),
+    "bec_projections": (C.c_int, [_P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P]),
+    "bec_pulse": (C.c_int, [_P, C.c_int]),
+    "bec_get_state": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "bec_set_state": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "bec_get_time": (C.c_double, [_P]),
+    "bec_set_time": (C.c_int, [_P, C.c_double]),
+    "bec_time_passes": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "bec_fft_plan3d": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bec_fft_execute": (C.c_int, [_P, _P, _P, C.c_int, _P]),
+    "bec_fft_destroy": (C.c_int, [_P]),
+    "bec_device_alloc": (C.c_int, [C.POINTER(_P), C.c_size_t]),
+    "bec_device_free": (C.c_int, [_P]),
+    "bec_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
+    "bec_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "bec_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libbec_amd.so; raise if it is not built (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m bec_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, where):
+    if status != OK:
+        raise BecError(status, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def default_model(**kw):
+    m = ModelParams()
+    check(lib().bec_default_model_params(C.byref(m)), "bec_default_model_params")
+    for k, v in kw.items():
+        if not hasattr(m, k):
+            raise KeyError(k)
+        setattr(m, k, v)
+    return m
+
+
+def derive(model):
+    c = CalcParams()
+    check(lib().bec_fill_derived_params(C.byref(model), C.byref(c)), "bec_fill_derived_params")
+    return c
+
+
+def gs_options(**kw):
+    o = GsOptions()
+    check(lib().bec_default_gs_options(C.byref(o)), "bec_default_gs_options")
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def device_count():
+    n = C.c_int(0)
+    st = lib().bec_device_count(C.byref(n))
+    return n.value if st == OK else 0
+
+
+def ground_states_batched(params, guesses=None, **opt):
+    """Batched imaginary-time ground states for a list of CalcParams sharing one lattice."""
+    L = lib()
+    B = len(params)
+    arr = (CalcParams * B)(*params)
+    o = gs_options(**opt)
+    cells = params[0].cells
+    out = np.empty((B, params[0].nvz, params[0].nvy, params[0].nvx), dtype=np.complex128)
+    reps = (GsReport * B)()
+    g = None if guesses is None else _c128(guesses)
+    st = L.bec_ground_states_batched(arr, B, None if g is None else _ptr(g), C.byref(o), None, _ptr(out), reps)
+    if st not in (OK, ERR_NOT_CONVERGED):
+        check(st, "bec_ground_states_batched")
+    assert out.size == B * cells
+    return out, [r.as_dict() for r in reps], st
+
+
+class Solver:
+    """Host-side mirror of the reference driver (programs/bec_wigner_3d.cuda/wigner.cpp:313-369):
+    calculateSteadyState -> initEvolution -> propagate / calculateEvolution -> observables."""
+
+    def __init__(self, calc, precision=FP32, groups=None):
+        L = lib()
+        self._ctx = C.c_void_p()
+        if groups is None:
+            groups = [calc]
+        self.groups = list(groups)
+        arr = (CalcParams * len(self.groups))(*self.groups)
+        check(L.bec_create_groups(arr, len(self.groups), precision, None, C.byref(self._ctx)), "bec_create")
+        self.precision = precision
+
+    def close(self):
+        if self._ctx:
+            lib().bec_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def shape(self, g=0):
+        c = self.groups[g]
+        return (c.ne, c.nvz, c.nvy, c.nvx)
+
+    def calculate_steady_state(self, group=0, guess=None, **opt):
+        c = self.groups[group]
+        o = gs_options(**opt)
+        out = np.empty((c.nvz, c.nvy, c.nvx), dtype=np.complex128)
+        rep = GsReport()
+        g = None if guess is None else _c128(guess)
+        st = lib().bec_calculate_steady_state(self._ctx, group, None if g is None else _ptr(g), C.byref(o),
+                                               _ptr(out), C.byref(rep))
+        if st not in (OK, ERR_NOT_CONVERGED):
+            check(st, "bec_calculate_steady_state")
+        return out, rep.as_dict(), st
+
+    def state_residual(self, psi, group=0):
+        r, mu = C.c_double(), C.c_double()
+        check(lib().bec_state_residual(self._ctx, group, _ptr(_c128(psi)), C.byref(r), C.byref(mu)),
+              "bec_state_residual")
+        return r.value, mu.value
+
+    def init_evolution(self, gs, seed, group=0, equilibrate=True):
+        check(lib().bec_init_evolution(self._ctx, group, _ptr(_c128(gs)), seed, 1 if equilibrate else 0),
+              "bec_init_evolution")
+
+    def propagate(self, n_steps, dt=0.0, sample_every=0):
+        ntot = sum(c.ne for c in self.groups)
+        ns = n_steps // sample_every if sample_every > 0 else 0
+        buf = np.empty((max(ns, 1), ntot, 4), dtype=np.float64)
+        check(lib().bec_propagate(self._ctx, n_steps, dt, sample_every, _ptr(buf) if ns else None),
+              "bec_propagate")
+        return buf[:ns]
+
+    def propagate_graph(self, n_steps, steps_per_graph, dt=0.0):
+        check(lib().bec_propagate_graph(self._ctx, n_steps, dt, steps_per_graph), "bec_propagate_graph")
+
+    def calculate_evolution(self, dt=0.0):
+        obs = (Obs * len(self.groups))()
+        check(lib().bec_calculate_evolution(self._ctx, dt, obs), "bec_calculate_evolution")
+        return [o.as_dict() for o in obs]
+
+    def observables(self, group=0):
+        o = Obs()
+        check(lib().bec_observables(self._ctx, group, C.byref(o)), "bec_observables")
+        return o.as_dict()
+
+    def trajectory_sums(self, group=0):
+        out = np.empty((self.groups[group].ne, 4), dtype=np.float64)
+        check(lib().bec_trajectory_sums(self._ctx, group, _ptr(out)), "bec_trajectory_sums")
+        return out
+
+    def projections(self, alpha=0.0, group=0):
+        c = self.groups[group]
+        xa = np.empty((c.nvy, c.nvx)); xb = np.empty_like(xa)
+        za = np.empty((c.nvy, c.nvz)); zb = np.empty_like(za)
+        aa = np.empty(c.nvz); ab = np.empty_like(aa)
+        check(lib().bec_projections(self._ctx, group, alpha, *[_ptr(v) for v in (xa, xb, za, zb, aa, ab)]),
+              "bec_projections")
+        return dict(xy_a=xa, xy_b=xb, zy_a=za, zy_b=zb, axial_a=aa, axial_b=ab)
+
+    def pulse(self, kind=PULSE_HALF_PI):
+        check(lib().bec_pulse(self._ctx, kind), "bec_pulse")
+
+    def get_state(self, space=SPACE_X, group=0):
+        a = np.empty(self.shape(group), dtype=np.complex128)
+        b = np.empty_like(a)
+        check(lib().bec_get_state(self._ctx, group, space, _ptr(a), _ptr(b)), "bec_get_state")
+        return a, b
+
+    def set_state(self, a, b, space=SPACE_X, group=0):
+        check(lib().bec_set_state(self._ctx, group, space, _ptr(_c128(a)), _ptr(_c128(b))), "bec_set_state")
+
+    @property
+    def t(self):
+        return lib().bec_get_time(self._ctx)
+
+    @t.setter
+    def t(self, v):
+        check(lib().bec_set_time(self._ctx, v), "bec_set_time")
+
+    def time_passes(self, iters=20, group=0):
+        ms = np.zeros(3)
+        check(lib().bec_time_passes(self._ctx, group, iters, _ptr(ms)), "bec_time_passes")
+        return dict(z_pass=ms[0], xy_pass=ms[1], sample_pass=ms[2])
+
+    def synchronize(self):
+        check(lib().bec_synchronize(self._ctx), "bec_synchronize")
+
+
+class FFTPlan:
+    """batchfftPlan3d / batchfftExecute / batchfftDestroy (programs/bec_wigner_3d.cuda/batchfft.h:83-108)."""
+
+    def __init__(self, nz, ny, nx, batch, precision=FP32):
+        self._p = C.c_void_p()
+        check(lib().bec_fft_plan3d(C.byref(self._p), nz, ny, nx, precision, batch), "bec_fft_plan3d")
+        self.shape = (batch, nz, ny, nx)
+        self.precision = precision
+
+    def execute(self, data, sign):
+        """Transform a host array through device memory (test helper)."""
+        L = lib()
+        dt = np.complex64 if self.precision == FP32 else np.complex128
+        h = np.ascontiguousarray(data, dtype=dt).reshape(self.shape)
+        d = C.c_void_p()
+        check(L.bec_device_alloc(C.byref(d), h.nbytes), "bec_device_alloc")
+        try:
+            check(L.bec_memcpy_h2d(d, _ptr(h), h.nbytes), "bec_memcpy_h2d")
+            check(L.bec_fft_execute(self._p, d, d, sign, None), "bec_fft_execute")
+            out = np.empty_like(h)
+            check(L.bec_memcpy_d2h(_ptr(out), d, h.nbytes), "bec_memcpy_d2h")
+        finally:
+            L.bec_device_free(d)
+        return out
+
+    def close(self):
+        if self._p:
+            lib().bec_fft_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

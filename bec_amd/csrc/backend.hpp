@@ -1,0 +1,58 @@
+// Internal interface between the host orchestration (bec_api.cpp) and the
+// kernel launchers (kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace bec {
+
+// One group of trajectories sharing a lattice shape.  Trajectory `traj` uses
+// parameter record traj >> rec_shift.
+struct DevGroup {
+    int prec;            // 0: complex64 state, 1: complex128 state
+    int nx, ny, nz;
+    int ntraj;
+    int rec_shift;
+    int n_records;
+    void *a, *b;         // state arrays, ntraj * cells complex each
+    TrajParams *params;  // device records [n_records]
+    void *kxy;           // kinetic tables [KT_COUNT][n_records][nx*ny] complex
+    void *kz;            // [KT_COUNT][n_records][nz] complex
+    void *tw;            // z twiddles exp(+2 pi i j / nz), j < nz, complex
+    int pw;              // pencils per z-pass workgroup (0: default)
+};
+
+enum { KT_RT_HALF = 0, KT_RT_FULL = 1, KT_IM_HALF = 2, KT_COUNT = 3 };
+
+bool shape_supported(int nx, int ny, int nz);
+size_t elem_size(int prec);                       // bytes of one complex value
+long xy_workgroups(const DevGroup &g, int ntraj); // partial-sum slots of an xy pass
+int xy_wg_per_traj(const DevGroup &g);
+int default_pw(int prec, int nx, int nz);
+
+hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s);
+hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s);
+hipError_t run_xy_sample(const DevGroup &g, double *partial, hipStream_t s);
+hipError_t run_reduce_partials(const double *partial, int per_traj, int ntraj, int nsum,
+                               double *out, hipStream_t s);
+// 2D transform of every plane of `ntraj` trajectories (in -> out), times scale
+hipError_t run_xy_fft(const DevGroup &g, const void *in, void *out, int ntraj, int sign,
+                      double scale, hipStream_t s);
+// transform along z of every pencil (in -> out), times scale
+hipError_t run_z_fft(const DevGroup &g, const void *in, void *out, int ntraj, int sign,
+                     double scale, hipStream_t s);
+// x-space a, b = psi_gs + Va eta_a / sqrt(dV), Vb eta_b / sqrt(dV)
+hipError_t run_noise_init(const DevGroup &g, const void *gs, uint64_t seed, double Va, double Vb,
+                          double inv_sqrt_dv, hipStream_t s);
+hipError_t run_pulse(const DevGroup &g, int kind, hipStream_t s);
+// per-cell ensemble sums of vacuum-subtracted densities after halfPiRotate(alpha):
+// out[cell*2 + c], x-space
+hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, double *out,
+                                 hipStream_t s);
+hipError_t run_convert(int prec_from, const void *in, int prec_to, void *out, long n, hipStream_t s);
+
+} // namespace bec

@@ -1,0 +1,828 @@
+// C ABI: contexts, evolution, observables, checkpointing, FFT sub-API.
+//
+// Host orchestration of the reference's driver functions
+// (programs/bec_wigner_3d.cuda/wigner.cu): initEvolution, propagate,
+// calculateEvolution and the observables, on the two-pass kernel scheme of
+// passes.hpp.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host_common.hpp"
+
+struct bec_ctx {
+    int prec = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<bec::Group> groups;
+    double t = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *d_samples = nullptr;
+    size_t sample_bytes = 0;
+};
+
+namespace bec {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_status(hipError_t e)
+{
+    if (e == hipSuccess) return BEC_OK;
+    if (e == hipErrorOutOfMemory) return BEC_ERR_ALLOC;
+    if (e == hipErrorInvalidValue || e == hipErrorInvalidConfiguration) return BEC_ERR_SHAPE;
+    return BEC_ERR_HIP;
+}
+
+hipError_t dev_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    return hipMalloc(p, bytes);
+}
+
+void dev_free(void *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+double k_of(int i, double dk, int n) { return (2 * i > n) ? dk * (i - n) : dk * i; }  // getK, wigner_kernel.cu:15-21
+
+TrajParams make_traj_params(const bec_calc_params &c, double dt)
+{
+    TrajParams p{};
+    p.xmax = c.xmax;
+    p.ymax = c.ymax;
+    p.zmax = c.zmax;
+    p.dx = c.dx;
+    p.dy = c.dy;
+    p.dz = c.dz;
+    p.inv_lambda2 = 1.0 / (c.lambda * c.lambda);
+    p.g11 = c.g11;
+    p.g12 = c.g12;
+    p.g22 = c.g22;
+    p.l111 = c.l111;
+    p.l12 = c.l12;
+    p.l22 = c.l22;
+    p.detuning = c.detuning;
+    p.dt = dt;
+    p.dt_steady = c.dt_steady;
+    p.N0 = (double)c.N;
+    p.vac = c.V / (2.0 * c.dx * c.dy * c.dz);
+    p.itmax = c.itmax;
+    return p;
+}
+
+static void put_complex(std::vector<char> &buf, long idx, int prec, double re, double im)
+{
+    if (prec == 0) {
+        float *f = (float *)buf.data();
+        f[2 * idx] = (float)re;
+        f[2 * idx + 1] = (float)im;
+    } else {
+        double *d = (double *)buf.data();
+        d[2 * idx] = re;
+        d[2 * idx + 1] = im;
+    }
+}
+
+// Kinetic tables of variant v for record 0: separable factors, computed in
+// double and rounded once.  KT_RT_*: exp(+i c dt k^2/2) with c = 1/2, 1
+// (wigner_kernel.cu:137-151, sign quirk Q1 kept); KT_IM_HALF:
+// exp(-dt_steady/2 * k^2/2) (wigner_kernel.cu:128-134).  1/nz is folded into
+// the z factor.
+int group_build_tables(Group &g, double dt, hipStream_t s)
+{
+    const bec_calc_params &c = g.calc;
+    const int nx = c.nvx, ny = c.nvy, nz = c.nvz, nxy = nx * ny, prec = g.dev.prec;
+    const size_t es = elem_size(prec);
+    const int R = g.dev.n_records;
+    std::vector<char> hxy((size_t)KT_COUNT * R * nxy * es), hz((size_t)KT_COUNT * R * nz * es);
+    for (int v = 0; v < KT_COUNT; ++v)
+        for (int r = 0; r < R; ++r) {
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    double kx = k_of(x, c.dkx, nx), ky = k_of(y, c.dky, ny);
+                    double e = (kx * kx + ky * ky) / 2.0;
+                    long idx = ((long)v * R + r) * nxy + y * nx + x;
+                    if (v == KT_IM_HALF) {
+                        put_complex(hxy, idx, prec, exp(-c.dt_steady / 2.0 * e), 0.0);
+                    } else {
+                        double ph = e * dt * (v == KT_RT_HALF ? 0.5 : 1.0);
+                        put_complex(hxy, idx, prec, cos(ph), sin(ph));
+                    }
+                }
+            for (int z = 0; z < nz; ++z) {
+                double kz = k_of(z, c.dkz, nz);
+                double e = kz * kz / 2.0;
+                long idx = ((long)v * R + r) * nz + z;
+                if (v == KT_IM_HALF) {
+                    put_complex(hz, idx, prec, exp(-c.dt_steady / 2.0 * e) / nz, 0.0);
+                } else {
+                    double ph = e * dt * (v == KT_RT_HALF ? 0.5 : 1.0);
+                    put_complex(hz, idx, prec, cos(ph) / nz, sin(ph) / nz);
+                }
+            }
+        }
+    BEC_HIP(hipMemcpyAsync(g.dev.kxy, hxy.data(), hxy.size(), hipMemcpyHostToDevice, s));
+    BEC_HIP(hipMemcpyAsync(g.dev.kz, hz.data(), hz.size(), hipMemcpyHostToDevice, s));
+    g.hp = make_traj_params(c, dt);
+    BEC_HIP(hipMemcpyAsync(g.dev.params, &g.hp, sizeof(TrajParams), hipMemcpyHostToDevice, s));
+    BEC_HIP(hipStreamSynchronize(s));
+    g.tables_dt = dt;
+    g.tables_dts = c.dt_steady;
+    return BEC_OK;
+}
+
+int group_setup(Group &g, const bec_calc_params &c, int prec, int ntraj, bool alloc_state)
+{
+    if (!shape_supported(c.nvx, c.nvy, c.nvz)) {
+        set_last_error("unsupported lattice: need nvx == nvy in {16, 32}, nvz in {64, 128, 256}");
+        return BEC_ERR_SHAPE;
+    }
+    if (ntraj <= 0) return BEC_ERR_INVALID;
+    g.calc = c;
+    g.cells = (long)c.nvx * c.nvy * c.nvz;
+    DevGroup &d = g.dev;
+    d.prec = prec;
+    d.nx = c.nvx;
+    d.ny = c.nvy;
+    d.nz = c.nvz;
+    d.ntraj = ntraj;
+    d.rec_shift = 30;
+    d.n_records = 1;
+    d.pw = 0;
+    const size_t es = elem_size(prec);
+    if (alloc_state) {
+        BEC_HIP(dev_alloc(&d.a, es * g.cells * ntraj));
+        BEC_HIP(dev_alloc(&d.b, es * g.cells * ntraj));
+    }
+    BEC_HIP(dev_alloc((void **)&d.params, sizeof(TrajParams)));
+    BEC_HIP(dev_alloc(&d.kxy, es * KT_COUNT * c.nvx * c.nvy));
+    BEC_HIP(dev_alloc(&d.kz, es * KT_COUNT * c.nvz));
+    BEC_HIP(dev_alloc(&d.tw, es * c.nvz));
+    std::vector<char> tw(es * c.nvz);
+    for (int j = 0; j < c.nvz; ++j) {
+        double ang = 2.0 * M_PI * j / c.nvz;
+        put_complex(tw, j, prec, cos(ang), sin(ang));
+    }
+    BEC_HIP(hipMemcpy(d.tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
+    g.partial_slots = xy_workgroups(d, ntraj);
+    BEC_HIP(dev_alloc((void **)&g.d_partial, sizeof(double) * 8 * g.partial_slots));
+    BEC_HIP(dev_alloc((void **)&g.d_sums, sizeof(double) * 4 * ntraj));
+    return BEC_OK;
+}
+
+void group_release(Group &g)
+{
+    DevGroup &d = g.dev;
+    dev_free(d.a);
+    dev_free(d.b);
+    void *p = d.params;
+    dev_free(p);
+    d.params = nullptr;
+    dev_free(d.kxy);
+    dev_free(d.kz);
+    dev_free(d.tw);
+    void *q = g.d_partial;
+    dev_free(q);
+    g.d_partial = nullptr;
+    void *r = g.d_sums;
+    dev_free(r);
+    g.d_sums = nullptr;
+}
+
+int upload_complex(const double *h, void *d, long n, int prec, hipStream_t s)
+{
+    if (prec == 1) {
+        BEC_HIP(hipMemcpyAsync(d, h, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+        BEC_HIP(hipStreamSynchronize(s));
+        return BEC_OK;
+    }
+    std::vector<float> f(2 * n);
+    for (long i = 0; i < 2 * n; ++i) f[i] = (float)h[i];
+    BEC_HIP(hipMemcpyAsync(d, f.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+    BEC_HIP(hipStreamSynchronize(s));
+    return BEC_OK;
+}
+
+int download_complex(const void *d, double *h, long n, int prec, hipStream_t s)
+{
+    if (prec == 1) {
+        BEC_HIP(hipMemcpyAsync(h, d, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+        BEC_HIP(hipStreamSynchronize(s));
+        return BEC_OK;
+    }
+    std::vector<float> f(2 * n);
+    BEC_HIP(hipMemcpyAsync(f.data(), d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+    BEC_HIP(hipStreamSynchronize(s));
+    for (long i = 0; i < 2 * n; ++i) h[i] = f[i];
+    return BEC_OK;
+}
+
+} // namespace bec
+
+using namespace bec;
+
+#define CHECK_CTX(ctx)                                                                                 \
+    do {                                                                                               \
+        if (!(ctx)) return BEC_ERR_INVALID;                                                            \
+    } while (0)
+#define CHECK_GROUP(ctx, g)                                                                            \
+    do {                                                                                               \
+        CHECK_CTX(ctx);                                                                                \
+        if ((g) < 0 || (g) >= (int)(ctx)->groups.size()) return BEC_ERR_INVALID;                      \
+    } while (0)
+
+extern "C" const char *bec_strerror(int status)
+{
+    switch (status) {
+    case BEC_OK: return "ok";
+    case BEC_ERR_INVALID: return "invalid argument";
+    case BEC_ERR_SHAPE: return "unsupported grid shape or batch";
+    case BEC_ERR_ALLOC: return "allocation failed";
+    case BEC_ERR_HIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
+    case BEC_ERR_NOT_CONVERGED: return "ground state did not converge within max_iter";
+    case BEC_ERR_NONFINITE: return "non-finite value detected in the state";
+    case BEC_ERR_STATE: return "call out of order";
+    case BEC_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+extern "C" int bec_device_count(int *count)
+{
+    if (!count) return BEC_ERR_INVALID;
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return BEC_ERR_HIP;
+    }
+    return BEC_OK;
+}
+
+extern "C" int bec_create_groups(const bec_calc_params *groups, int n_groups, int precision, void *stream,
+                                 bec_ctx **out)
+{
+    if (!groups || n_groups <= 0 || !out) return BEC_ERR_INVALID;
+    if (precision != BEC_FP32 && precision != BEC_FP64) return BEC_ERR_INVALID;
+    *out = nullptr;
+    bec_ctx *ctx = new bec_ctx();
+    ctx->prec = precision;
+    if (stream) {
+        ctx->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            return hip_status(e);
+        }
+        ctx->own_stream = true;
+    }
+    ctx->groups.resize(n_groups);
+    for (int i = 0; i < n_groups; ++i) {
+        int st = group_setup(ctx->groups[i], groups[i], precision, groups[i].ne, true);
+        if (st == BEC_OK) st = group_build_tables(ctx->groups[i], groups[i].dt_evo, ctx->stream);
+        if (st != BEC_OK) {
+            bec_destroy(ctx);
+            return st;
+        }
+    }
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        bec_destroy(ctx);
+        return BEC_ERR_HIP;
+    }
+    *out = ctx;
+    return BEC_OK;
+}
+
+extern "C" int bec_create(const bec_calc_params *p, int precision, void *stream, bec_ctx **out)
+{
+    return bec_create_groups(p, 1, precision, stream, out);
+}
+
+extern "C" int bec_destroy(bec_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &g : ctx->groups) group_release(g);
+    void *p = ctx->d_samples;
+    dev_free(p);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BEC_OK;
+}
+
+extern "C" int bec_synchronize(bec_ctx *ctx)
+{
+    CHECK_CTX(ctx);
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+extern "C" int bec_group_count(const bec_ctx *ctx) { return ctx ? (int)ctx->groups.size() : 0; }
+extern "C" double bec_get_time(const bec_ctx *ctx) { return ctx ? ctx->t : 0.0; }
+extern "C" int bec_set_time(bec_ctx *ctx, double t)
+{
+    CHECK_CTX(ctx);
+    ctx->t = t;
+    return BEC_OK;
+}
+
+// ------------------------------------------------------------ evolution
+
+static int ensure_tables(bec_ctx *ctx, double dt)
+{
+    for (auto &g : ctx->groups)
+        if (g.tables_dt != dt) {
+            int st = group_build_tables(g, dt, ctx->stream);
+            if (st != BEC_OK) return st;
+        }
+    return BEC_OK;
+}
+
+static int sample_group(bec_ctx *ctx, Group &g, double *dst)
+{
+    BEC_HIP(run_xy_sample(g.dev, g.d_partial, ctx->stream));
+    BEC_HIP(run_reduce_partials(g.d_partial, xy_wg_per_traj(g.dev), g.dev.ntraj, 4, dst, ctx->stream));
+    return BEC_OK;
+}
+
+static int total_traj(const bec_ctx *ctx)
+{
+    int n = 0;
+    for (auto &g : ctx->groups) n += g.dev.ntraj;
+    return n;
+}
+
+// One call: K/2 N (K N)^(n-1) K/2 with K/2 + sample + K/2 at sample points.
+static int propagate_impl(bec_ctx *ctx, int n_steps, double dt, int sample_every, double *d_samples)
+{
+    bool fresh = true;  // next kinetic factor is a leading half step
+    int sample_idx = 0;
+    const int ntot = total_traj(ctx);
+    for (int s = 0; s < n_steps; ++s) {
+        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, fresh ? KT_RT_HALF : KT_RT_FULL, ctx->stream));
+        for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+        const bool sample = sample_every > 0 && (s + 1) % sample_every == 0;
+        const bool last = s == n_steps - 1;
+        if (sample || last) {
+            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+            if (sample && d_samples) {
+                long off = (long)sample_idx * ntot * 4;
+                for (auto &g : ctx->groups) {
+                    int st = sample_group(ctx, g, d_samples + off);
+                    if (st != BEC_OK) return st;
+                    off += (long)g.dev.ntraj * 4;
+                }
+            }
+            if (sample) ++sample_idx;
+            fresh = true;
+        } else {
+            fresh = false;
+        }
+    }
+    return BEC_OK;
+}
+
+extern "C" int bec_propagate(bec_ctx *ctx, int n_steps, double dt, int sample_every, double *sums_out)
+{
+    CHECK_CTX(ctx);
+    if (n_steps < 0) return BEC_ERR_INVALID;
+    for (auto &g : ctx->groups)
+        if (!g.initialized) return BEC_ERR_STATE;
+    if (n_steps == 0) return BEC_OK;
+    if (dt <= 0) dt = ctx->groups[0].calc.dt_evo;
+    int st = ensure_tables(ctx, dt);
+    if (st != BEC_OK) return st;
+    const int n_samples = sample_every > 0 ? n_steps / sample_every : 0;
+    const size_t bytes = sizeof(double) * 4 * (size_t)total_traj(ctx) * n_samples;
+    if (n_samples > 0 && sums_out && bytes > ctx->sample_bytes) {
+        void *p = ctx->d_samples;
+        dev_free(p);
+        ctx->d_samples = nullptr;
+        BEC_HIP(dev_alloc((void **)&ctx->d_samples, bytes));
+        ctx->sample_bytes = bytes;
+    }
+    st = propagate_impl(ctx, n_steps, dt, sums_out ? sample_every : 0, sums_out ? ctx->d_samples : nullptr);
+    if (st != BEC_OK) return st;
+    ctx->t += n_steps * dt;
+    if (n_samples > 0 && sums_out) {
+        BEC_HIP(hipMemcpyAsync(sums_out, ctx->d_samples, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+extern "C" int bec_propagate_graph(bec_ctx *ctx, int n_steps, double dt, int steps_per_graph)
+{
+    CHECK_CTX(ctx);
+    if (n_steps < 0 || steps_per_graph <= 0 || n_steps % steps_per_graph) return BEC_ERR_INVALID;
+    for (auto &g : ctx->groups)
+        if (!g.initialized) return BEC_ERR_STATE;
+    if (n_steps == 0) return BEC_OK;
+    if (dt <= 0) dt = ctx->groups[0].calc.dt_evo;
+    int st = ensure_tables(ctx, dt);
+    if (st != BEC_OK) return st;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    st = propagate_impl(ctx, steps_per_graph, dt, 0, nullptr);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != BEC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    BEC_HIP(e);
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        BEC_HIP(e);
+    }
+    for (int r = 0; r < n_steps / steps_per_graph; ++r) {
+        e = hipGraphLaunch(exec, ctx->stream);
+        if (e != hipSuccess) break;
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    BEC_HIP(e);
+    BEC_HIP(e2);
+    ctx->t += n_steps * dt;
+    return BEC_OK;
+}
+
+extern "C" int bec_pulse(bec_ctx *ctx, int kind)
+{
+    CHECK_CTX(ctx);
+    if (kind != BEC_PULSE_HALF_PI && kind != BEC_PULSE_PI) return BEC_ERR_INVALID;
+    for (auto &g : ctx->groups) BEC_HIP(run_pulse(g.dev, kind, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+// The equilibration loop of initEvolution (wigner.cu:222-223) is written in
+// float: for(t = 0; t <= t_equilib; t += dt_evo).  It runs at least once.
+static int equilibration_steps(const bec_calc_params &c)
+{
+    float te = (float)c.t_equilib, dt = (float)c.dt_evo;
+    if (!(dt > 0.0f)) return 1;
+    int n = 0;
+    for (float t = 0.0f; t <= te; t += dt) ++n;
+    return n;
+}
+
+extern "C" int bec_init_evolution(bec_ctx *ctx, int group, const double *gs_x, uint64_t seed, int equilibrate)
+{
+    CHECK_GROUP(ctx, group);
+    if (!gs_x) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    const size_t es = elem_size(g.dev.prec);
+    void *d_gs = nullptr;
+    BEC_HIP(dev_alloc(&d_gs, es * g.cells));
+    int st = upload_complex(gs_x, d_gs, g.cells, g.dev.prec, ctx->stream);
+    if (st == BEC_OK) {
+        const bec_calc_params &c = g.calc;
+        hipError_t e = run_noise_init(g.dev, d_gs, seed, c.Va, c.Vb, 1.0 / sqrt(c.dx * c.dy * c.dz), ctx->stream);
+        if (e == hipSuccess) {
+            // x-space -> hybrid: transform along x, y with 1/(nx ny)
+            const double sc = 1.0 / (c.nvx * c.nvy);
+            e = run_xy_fft(g.dev, g.dev.a, g.dev.a, g.dev.ntraj, +1, sc, ctx->stream);
+            if (e == hipSuccess) e = run_xy_fft(g.dev, g.dev.b, g.dev.b, g.dev.ntraj, +1, sc, ctx->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            st = hip_status(e);
+        }
+    }
+    dev_free(d_gs);
+    if (st != BEC_OK) return st;
+    g.initialized = true;
+    if (equilibrate) {
+        // equilibration runs on this group only
+        const int n = equilibration_steps(g.calc);
+        st = ensure_tables(ctx, g.calc.dt_evo);
+        if (st != BEC_OK) return st;
+        for (int s = 0; s < n; ++s) {
+            BEC_HIP(run_z_kinetic(g.dev, s == 0 ? KT_RT_HALF : KT_RT_FULL, ctx->stream));
+            BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+        }
+        BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+    }
+    BEC_HIP(run_pulse(g.dev, BEC_PULSE_HALF_PI, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+// ---------------------------------------------------------- observables
+
+extern "C" int bec_trajectory_sums(bec_ctx *ctx, int group, double *sums)
+{
+    CHECK_GROUP(ctx, group);
+    if (!sums) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    int st = sample_group(ctx, g, g.d_sums);
+    if (st != BEC_OK) return st;
+    BEC_HIP(hipMemcpyAsync(sums, g.d_sums, sizeof(double) * 4 * g.dev.ntraj, hipMemcpyDeviceToHost, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+// Observables from the per-trajectory sums S_aa, S_bb, S_ab = sum conj(a) b.
+// halfPiRotate(alpha) (wigner_kernel.cu:344-360) gives per cell
+//   |a'|^2 = (|a|^2 + |b|^2)/2 + Re(conj(a) b (sin - i cos)) ,
+//   |b'|^2 = (|a|^2 + |b|^2)/2 - Re(conj(a) b (sin - i cos)) ... with
+// Re(conj(a) b (s - i c)) = s Re(conj(a) b) + c Im(conj(a) b), so the sums
+// after any virtual pulse follow from the four sums exactly.
+static void obs_from_sums(const bec_calc_params &c, const double *sums, int ne, double t, bec_obs *o)
+{
+    double saa = 0, sbb = 0, sre = 0, sim = 0;
+    for (int e = 0; e < ne; ++e) {
+        saa += sums[4 * e];
+        sbb += sums[4 * e + 1];
+        sre += sums[4 * e + 2];
+        sim += sums[4 * e + 3];
+    }
+    const double dV = c.dx * c.dy * c.dz;
+    const double vac_total = (double)ne * c.cells * c.V / (2.0 * dV);
+    auto rotated = [&](double alpha, double *n1, double *n2) {
+        const double cr = sin(alpha) * sre + cos(alpha) * sim;
+        *n1 = 0.5 * (saa + sbb) + cr - vac_total;
+        *n2 = 0.5 * (saa + sbb) - cr - vac_total;
+    };
+    double n1, n2;
+    rotated(0.0, &n1, &n2);
+    o->t = t;
+    o->Na = n1 * dV;
+    o->Nb = n2 * dV;
+    o->N_total = (n1 + n2) * dV / ne;
+    double vmax = 0;
+    for (float alpha = 0.0f; alpha < 2.0f * (float)M_PI; alpha += 0.5f) {   // getVisibility, wigner.cu:267-277
+        double a1, a2;
+        rotated((double)alpha, &a1, &a2);
+        a1 = fabs(a1);
+        a2 = fabs(a2);
+        double r = (a1 - a2) / (a1 + a2);
+        if (fabs(r) > vmax) vmax = fabs(r);
+    }
+    o->visibility_grid = vmax;
+    const double N1 = saa - vac_total, N2 = sbb - vac_total;
+    o->N1 = N1 * dV;
+    o->N2 = N2 * dV;
+    o->visibility = 2.0 * sqrt(sre * sre + sim * sim) / (N1 + N2);
+    o->phase = atan2(sim, sre);
+}
+
+extern "C" int bec_observables(bec_ctx *ctx, int group, bec_obs *out)
+{
+    CHECK_GROUP(ctx, group);
+    if (!out) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    std::vector<double> sums(4 * (size_t)g.dev.ntraj);
+    int st = bec_trajectory_sums(ctx, group, sums.data());
+    if (st != BEC_OK) return st;
+    for (double v : sums)
+        if (!std::isfinite(v)) return BEC_ERR_NONFINITE;
+    obs_from_sums(g.calc, sums.data(), g.dev.ntraj, ctx->t, out);
+    return BEC_OK;
+}
+
+extern "C" int bec_calculate_evolution(bec_ctx *ctx, double dt, bec_obs *out)
+{
+    CHECK_CTX(ctx);
+    int st = bec_propagate(ctx, 1, dt, 0, nullptr);
+    if (st != BEC_OK) return st;
+    if (out)
+        for (int i = 0; i < (int)ctx->groups.size(); ++i) {
+            st = bec_observables(ctx, i, &out[i]);
+            if (st != BEC_OK) return st;
+        }
+    return BEC_OK;
+}
+
+extern "C" int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy_a, double *xy_b, double *zy_a,
+                               double *zy_b, double *axial_a, double *axial_b)
+{
+    CHECK_GROUP(ctx, group);
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    const int nx = g.calc.nvx, ny = g.calc.nvy, nz = g.calc.nvz;
+    double *d_out = nullptr;
+    BEC_HIP(dev_alloc((void **)&d_out, sizeof(double) * 2 * g.cells));
+    const double vac = g.calc.V / (2.0 * g.calc.dx * g.calc.dy * g.calc.dz);
+    hipError_t e = run_cell_density_sums(g.dev, alpha, vac, d_out, ctx->stream);
+    std::vector<double> cs(2 * g.cells);
+    if (e == hipSuccess) e = hipMemcpyAsync(cs.data(), d_out, sizeof(double) * 2 * g.cells, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    void *p = d_out;
+    dev_free(p);
+    BEC_HIP(e);
+    // projections of wigner.cu:388-410 (xy: sum over z; zy stored [y][z]: sum
+    // over x) and the axial profile of wigner.cu:279-303 (sum over x, y)
+    for (int c = 0; c < 2; ++c) {
+        double *xy = c ? xy_b : xy_a, *zy = c ? zy_b : zy_a, *ax = c ? axial_b : axial_a;
+        if (xy) std::fill(xy, xy + nx * ny, 0.0);
+        if (zy) std::fill(zy, zy + ny * nz, 0.0);
+        if (ax) std::fill(ax, ax + nz, 0.0);
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    double v = cs[2 * (((long)z * ny + y) * nx + x) + c];
+                    if (xy) xy[y * nx + x] += v;
+                    if (zy) zy[y * nz + z] += v;
+                    if (ax) ax[z] += v;
+                }
+    }
+    return BEC_OK;
+}
+
+// ---------------------------------------------------------- checkpoint
+
+extern "C" int bec_get_state(bec_ctx *ctx, int group, int space, double *a, double *b)
+{
+    CHECK_GROUP(ctx, group);
+    if (space != BEC_SPACE_X && space != BEC_SPACE_K) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    const long n = g.cells * g.dev.ntraj;
+    void *tmp = nullptr;
+    BEC_HIP(dev_alloc(&tmp, elem_size(g.dev.prec) * n));
+    int st = BEC_OK;
+    for (int c = 0; c < 2 && st == BEC_OK; ++c) {
+        double *h = c ? b : a;
+        if (!h) continue;
+        void *src = c ? g.dev.b : g.dev.a;
+        hipError_t e = space == BEC_SPACE_X
+                           ? run_xy_fft(g.dev, src, tmp, g.dev.ntraj, -1, 1.0, ctx->stream)
+                           : run_z_fft(g.dev, src, tmp, g.dev.ntraj, +1, 1.0 / g.calc.nvz, ctx->stream);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            st = hip_status(e);
+            break;
+        }
+        st = download_complex(tmp, h, n, g.dev.prec, ctx->stream);
+    }
+    dev_free(tmp);
+    return st;
+}
+
+extern "C" int bec_set_state(bec_ctx *ctx, int group, int space, const double *a, const double *b)
+{
+    CHECK_GROUP(ctx, group);
+    if (space != BEC_SPACE_X && space != BEC_SPACE_K) return BEC_ERR_INVALID;
+    if (!a || !b) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    const long n = g.cells * g.dev.ntraj;
+    for (int c = 0; c < 2; ++c) {
+        void *dst = c ? g.dev.b : g.dev.a;
+        int st = upload_complex(c ? b : a, dst, n, g.dev.prec, ctx->stream);
+        if (st != BEC_OK) return st;
+        hipError_t e = space == BEC_SPACE_X
+                           ? run_xy_fft(g.dev, dst, dst, g.dev.ntraj, +1, 1.0 / (g.calc.nvx * g.calc.nvy), ctx->stream)
+                           : run_z_fft(g.dev, dst, dst, g.dev.ntraj, -1, 1.0, ctx->stream);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            return hip_status(e);
+        }
+    }
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    g.initialized = true;
+    return BEC_OK;
+}
+
+// -------------------------------------------------------------- timing
+
+extern "C" int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms)
+{
+    CHECK_GROUP(ctx, group);
+    if (iters <= 0 || !ms) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    int st = ensure_tables(ctx, g.tables_dt > 0 ? g.tables_dt : g.calc.dt_evo);
+    if (st != BEC_OK) return st;
+    float t = 0;
+    // z-pass (full kinetic step)
+    BEC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < iters; ++i) BEC_HIP(run_z_kinetic(g.dev, KT_RT_FULL, ctx->stream));
+    BEC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    BEC_HIP(hipEventSynchronize(ctx->ev1));
+    BEC_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    ms[0] = t / iters;
+    BEC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < iters; ++i) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+    BEC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    BEC_HIP(hipEventSynchronize(ctx->ev1));
+    BEC_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    ms[1] = t / iters;
+    BEC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < iters; ++i) BEC_HIP(run_xy_sample(g.dev, g.d_partial, ctx->stream));
+    BEC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    BEC_HIP(hipEventSynchronize(ctx->ev1));
+    BEC_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    ms[2] = t / iters;
+    return BEC_OK;
+}
+
+// ------------------------------------------------------------- FFT API
+
+struct bec_fft_plan {
+    bec::DevGroup dev{};
+    void *tw = nullptr;
+};
+
+extern "C" int bec_fft_plan3d(bec_fft_plan **plan, int nz, int ny, int nx, int precision, int batch)
+{
+    if (!plan || batch <= 0) return BEC_ERR_INVALID;
+    if (precision != BEC_FP32 && precision != BEC_FP64) return BEC_ERR_INVALID;
+    *plan = nullptr;
+    if (!shape_supported(nx, ny, nz)) return BEC_ERR_SHAPE;
+    bec_fft_plan *p = new bec_fft_plan();
+    p->dev.prec = precision;
+    p->dev.nx = nx;
+    p->dev.ny = ny;
+    p->dev.nz = nz;
+    p->dev.ntraj = batch;
+    const size_t es = elem_size(precision);
+    std::vector<char> tw(es * nz);
+    for (int j = 0; j < nz; ++j) {
+        double ang = 2.0 * M_PI * j / nz;
+        if (precision == 0) {
+            ((float *)tw.data())[2 * j] = (float)cos(ang);
+            ((float *)tw.data())[2 * j + 1] = (float)sin(ang);
+        } else {
+            ((double *)tw.data())[2 * j] = cos(ang);
+            ((double *)tw.data())[2 * j + 1] = sin(ang);
+        }
+    }
+    if (dev_alloc(&p->dev.tw, tw.size()) != hipSuccess ||
+        hipMemcpy(p->dev.tw, tw.data(), tw.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        dev_free(p->dev.tw);
+        delete p;
+        return BEC_ERR_ALLOC;
+    }
+    *plan = p;
+    return BEC_OK;
+}
+
+extern "C" int bec_fft_execute(bec_fft_plan *plan, const void *d_in, void *d_out, int sign, void *stream)
+{
+    if (!plan || !d_in || !d_out) return BEC_ERR_INVALID;
+    if (sign != BEC_FFT_FORWARD && sign != BEC_FFT_INVERSE) return BEC_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    BEC_HIP(run_xy_fft(plan->dev, d_in, d_out, plan->dev.ntraj, sign, 1.0, s));
+    BEC_HIP(run_z_fft(plan->dev, d_out, d_out, plan->dev.ntraj, sign, 1.0, s));
+    BEC_HIP(hipStreamSynchronize(s));
+    return BEC_OK;
+}
+
+extern "C" int bec_fft_destroy(bec_fft_plan *plan)
+{
+    if (!plan) return BEC_ERR_INVALID;
+    dev_free(plan->dev.tw);
+    delete plan;
+    return BEC_OK;
+}
+
+// --------------------------------------------------------- device memory
+
+extern "C" int bec_device_alloc(void **d_ptr, size_t bytes)
+{
+    if (!d_ptr) return BEC_ERR_INVALID;
+    BEC_HIP(hipMalloc(d_ptr, bytes));
+    return BEC_OK;
+}
+extern "C" int bec_device_free(void *d_ptr)
+{
+    BEC_HIP(hipFree(d_ptr));
+    return BEC_OK;
+}
+extern "C" int bec_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes)
+{
+    BEC_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    return BEC_OK;
+}
+extern "C" int bec_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes)
+{
+    BEC_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return BEC_OK;
+}
+
+namespace bec {
+const bec_calc_params *ctx_group_params(bec_ctx *ctx, int group, hipStream_t *stream)
+{
+    if (!ctx || group < 0 || group >= (int)ctx->groups.size()) return nullptr;
+    if (stream) *stream = ctx->stream;
+    return &ctx->groups[group].calc;
+}
+} // namespace bec

@@ -1,0 +1,63 @@
+// Ground-state kernels: interface between gs.cpp and kernels_gs.hip.
+#pragma once
+
+#include "backend.hpp"
+
+namespace bec {
+
+// Per-trajectory scalar state of a batched ground-state solve (device).
+struct GsScalars {
+    double *s;        // pending renormalisation factor
+    double *mu;       // Rayleigh quotient of the current iterate
+    double *e_prev;   // previous reference energy integral
+    double *res;      // last residual
+    double *e_ref;    // last reference energy integral (not per particle)
+    int *iters;       // iterations done
+    int *done;        // converged flag
+};
+
+// Energy tables: e_xy[r][q] = (kx^2 + ky^2)/2, e_z[r][kz] = kz^2/2 (double)
+struct GsTables {
+    const double *exy;
+    const double *ez;
+};
+
+// ---- split step (calculateSteadyState, wigner.cu:84-175)
+// z-pass with the imaginary-time half step (and the pending scale); when
+// kcopy != nullptr it also stores the normalised k-space values.
+hipError_t gs_z_imag(const DevGroup &g, void *psi, const GsScalars &sc, void *kcopy, hipStream_t s);
+hipError_t gs_xy_nonlinear_imag(const DevGroup &g, void *psi, const GsScalars &sc, hipStream_t s);
+// partials: {sum n, sum n V, sum n^2, sum Re(psi_x psi_k k^2/2)} per workgroup
+hipError_t gs_xy_measure_ref(const DevGroup &g, const void *psi, const void *kcopy, const GsScalars &sc,
+                             const GsTables &t, double *partial, hipStream_t s);
+// reference rule: s = sqrt(N0/N), E = integral of the renormalised state,
+// done when |(E - E_prev)/E| < tol
+hipError_t gs_scalar_ref(const DevGroup &g, const double *partial, const GsScalars &sc, double tol,
+                         int use_rule, hipStream_t s);
+
+// ---- gradient flow / residual machinery (x-space iterate psi)
+hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s);
+// variant 0: multiply by k^2/2 / nz; 1: by 1 / ((alpha + k^2/2) nz)
+hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTables &t, int variant,
+                     double alpha, hipStream_t s);
+// h = K psi + (V + g11 |psi|^2) psi; partials {Re<psi,h>, <psi,psi>, <psi,K psi>, sum V n, sum n^2}
+hipError_t gs_xy_hamiltonian(const DevGroup &g, const void *psi, void *w, const GsScalars &sc,
+                             double *partial, hipStream_t s);
+hipError_t gs_scalar_mu(const DevGroup &g, const double *partial, const GsScalars &sc, hipStream_t s);
+// r = h - mu psi; partial {|r|^2}; w <- F_xy(r)
+hipError_t gs_xy_residual(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, double *partial,
+                          hipStream_t s);
+hipError_t gs_scalar_residual(const DevGroup &g, const double *partial_h, const double *partial_r,
+                              const GsScalars &sc, double tol, int check, hipStream_t s);
+// psi_new = s psi - tau d + beta (s psi - prev); prev <- s psi; partial {|psi_new|^2}
+hipError_t gs_xy_update(const DevGroup &g, void *psi, void *prev, const void *w, const GsScalars &sc,
+                        double tau, double beta, double *partial, hipStream_t s);
+hipError_t gs_scalar_norm(const DevGroup &g, const double *partial, const GsScalars &sc, hipStream_t s);
+// s <- sqrt(N0 / (dV sum |psi|^2)) of an x-space state (no iteration count)
+hipError_t gs_norm_only(const DevGroup &g, const void *psi, const GsScalars &sc, double *partial, hipStream_t s);
+// psi <- s psi (x-space), s <- 1
+hipError_t gs_apply_scale(const DevGroup &g, void *psi, const GsScalars &sc, hipStream_t s);
+// Thomas-Fermi start (fillWithTFGroundState, wigner_kernel.cu:116-125), per record mu
+hipError_t gs_tf_start(const DevGroup &g, void *psi, const double *mu_tf, hipStream_t s);
+
+} // namespace bec

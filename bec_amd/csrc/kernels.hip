@@ -1,0 +1,321 @@
+// Kernel launchers for gfx950: dispatch from runtime lattice shape and
+// precision to the compile-time pass templates of passes.hpp.
+#include "launch.hpp"
+
+namespace bec {
+
+bool shape_supported(int nx, int ny, int nz)
+{
+    return nx == ny && (nx == 16 || nx == 32) && (nz == 64 || nz == 128 || nz == 256);
+}
+
+size_t elem_size(int prec) { return prec == 0 ? 8 : 16; }
+
+long xy_workgroups(const DevGroup &g, int ntraj) { return (long)ntraj * g.nz / (64 / g.nx); }
+int xy_wg_per_traj(const DevGroup &g) { return g.nz / (64 / g.nx); }
+
+int default_pw(int prec, int nx, int nz)
+{
+    // keep the LDS exchange tile at <= 64 KiB
+    int pw = 65536 / (nz * (int)elem_size(prec));
+    if (pw > 64) pw = 64;
+    if (pw > nx * nx) pw = nx * nx;
+    return pw;
+}
+
+// ------------------------------------------------------------------ passes
+
+hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s)
+{
+    int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
+    return dispatch(g, [&](auto Tt, auto Nt, auto NZt) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int NZ = decltype(NZt)::value;
+        constexpr int R1 = z_r1<NZ>();
+        const int nxy = g.nx * g.ny;
+        OpKinetic<T, NZ, R1, 2> op;
+        op.a = (cpx<T> *)g.a;
+        op.b = (cpx<T> *)g.b;
+        op.kxy = (const cpx<T> *)g.kxy + (long)variant * g.n_records * nxy;
+        op.kz = (const cpx<T> *)g.kz + (long)variant * g.n_records * NZ;
+        op.nxy = nxy;
+        op.rec_shift = g.rec_shift;
+        return with_pw(pw, [&](auto PWt) {
+            return launch_z<T, NZ, decltype(PWt)::value, +1, true>(g, op, g.ntraj, s);
+        });
+    });
+}
+
+hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s)
+{
+    return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int N = decltype(Nt)::value;
+        OpNonlinRT<T, N> op;
+        op.a = (cpx<T> *)g.a;
+        op.b = (cpx<T> *)g.b;
+        op.P = g.params;
+        op.rec_shift = g.rec_shift;
+        return launch_xy<T, N>(g, op, g.ntraj, nullptr, s);
+    });
+}
+
+hipError_t run_xy_sample(const DevGroup &g, double *partial, hipStream_t s)
+{
+    return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int N = decltype(Nt)::value;
+        OpSample<T, N> op;
+        op.a = (const cpx<T> *)g.a;
+        op.b = (const cpx<T> *)g.b;
+        return launch_xy<T, N>(g, op, g.ntraj, partial, s);
+    });
+}
+
+hipError_t run_xy_fft(const DevGroup &g, const void *in, void *out, int ntraj, int sign, double scale,
+                      hipStream_t s)
+{
+    return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int N = decltype(Nt)::value;
+        if (sign < 0) {
+            OpFftXY<T, N, -1> op;
+            op.in = (const cpx<T> *)in;
+            op.out = (cpx<T> *)out;
+            op.scale = (T)scale;
+            return launch_xy<T, N>(g, op, ntraj, nullptr, s);
+        }
+        OpFftXY<T, N, +1> op;
+        op.in = (const cpx<T> *)in;
+        op.out = (cpx<T> *)out;
+        op.scale = (T)scale;
+        return launch_xy<T, N>(g, op, ntraj, nullptr, s);
+    });
+}
+
+hipError_t run_z_fft(const DevGroup &g, const void *in, void *out, int ntraj, int sign, double scale,
+                     hipStream_t s)
+{
+    int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
+    return dispatch(g, [&](auto Tt, auto, auto NZt) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int NZ = decltype(NZt)::value;
+        constexpr int R1 = z_r1<NZ>();
+        OpScaleZ<T, NZ, R1> op;
+        op.in = (const cpx<T> *)in;
+        op.out = (cpx<T> *)out;
+        op.scale = (T)scale;
+        return with_pw(pw, [&](auto PWt) {
+            constexpr int PW = decltype(PWt)::value;
+            if (sign < 0) return launch_z<T, NZ, PW, -1, false>(g, op, ntraj, s);
+            return launch_z<T, NZ, PW, +1, false>(g, op, ntraj, s);
+        });
+    });
+}
+
+// ------------------------------------------------------------- reductions
+
+__global__ void reduce_partials_kernel(const double *__restrict__ partial, int per_traj, int ntraj,
+                                       int nsum, double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntraj * nsum) return;
+    int traj = i / nsum, k = i % nsum;
+    const double *p = partial + (long)traj * per_traj * nsum + k;
+    double s = 0.0;
+    for (int j = 0; j < per_traj; ++j) s += p[(long)j * nsum];
+    out[i] = s;
+}
+
+hipError_t run_reduce_partials(const double *partial, int per_traj, int ntraj, int nsum, double *out,
+                               hipStream_t s)
+{
+    int n = ntraj * nsum;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial, per_traj,
+                       ntraj, nsum, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ noise init
+// Philox4x32-10 (Salmon et al., SC'11) with counter {cell, traj, 0, 0} and
+// key = seed; Box-Muller of fillWithNormalDistribution (wigner.cu:15-34,
+// dev 0.5) evaluated in FP64; initializeEnsembles (wigner_kernel.cu:295-317).
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+    }
+}
+
+template <typename T>
+__global__ void noise_init_kernel(const cpx<T> *__restrict__ gs, cpx<T> *__restrict__ a, cpx<T> *__restrict__ b,
+                                  long cells, long n, uint32_t k0, uint32_t k1, double Va, double Vb,
+                                  double coeff)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long cell = i % cells;
+    const long traj = i / cells;
+    uint32_t c[4] = {(uint32_t)cell, (uint32_t)traj, 0u, 0u};
+    philox4x32_10(c, k0, k1);
+    double eta[4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const double u1 = ((double)c[2 * p] + 1.0) * (1.0 / 4294967296.0);
+        const double u2 = ((double)c[2 * p + 1] + 1.0) * (1.0 / 4294967296.0);
+        const double rad = 0.5 * sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincospi(2.0 * u2, &sn, &cs);
+        eta[2 * p] = rad * cs;
+        eta[2 * p + 1] = rad * sn;
+    }
+    const cpx<T> g = gs[cell];
+    a[i] = mk<T>((T)((double)g.x + Va * eta[0] * coeff), (T)((double)g.y + Va * eta[1] * coeff));
+    b[i] = mk<T>((T)(Vb * eta[2] * coeff), (T)(Vb * eta[3] * coeff));
+}
+
+hipError_t run_noise_init(const DevGroup &g, const void *gs, uint64_t seed, double Va, double Vb,
+                          double coeff, hipStream_t s)
+{
+    long cells = (long)g.nx * g.ny * g.nz, n = cells * g.ntraj;
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (g.prec == 0)
+        hipLaunchKernelGGL(noise_init_kernel<float>, grid, block, 0, s, (const float2 *)gs, (float2 *)g.a,
+                           (float2 *)g.b, cells, n, k0, k1, Va, Vb, coeff);
+    else
+        hipLaunchKernelGGL(noise_init_kernel<double>, grid, block, 0, s, (const double2 *)gs, (double2 *)g.a,
+                           (double2 *)g.b, cells, n, k0, k1, Va, Vb, coeff);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------- pulses
+// applyHalfPiPulse / applyPiPulse (wigner_kernel.cu:320-341); pointwise and
+// linear, so valid in any linear representation of the state.
+template <typename T>
+__global__ void pulse_kernel(cpx<T> *__restrict__ a, cpx<T> *__restrict__ b, long n, int kind)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cpx<T> a0 = a[i], b0 = b[i];
+    if (kind == 0) {
+        const T r = (T)0.70710678118654752440;
+        a[i] = mk<T>((a0.x + b0.y) * r, (a0.y - b0.x) * r);   // (a - i b)/sqrt2
+        b[i] = mk<T>((a0.y + b0.x) * r, (b0.y - a0.x) * r);   // (-i a + b)/sqrt2
+    } else {
+        a[i] = mk<T>(b0.y, -b0.x);                            // -i b
+        b[i] = mk<T>(a0.y, -a0.x);                            // -i a
+    }
+}
+
+hipError_t run_pulse(const DevGroup &g, int kind, hipStream_t s)
+{
+    long n = (long)g.nx * g.ny * g.nz * g.ntraj;
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (g.prec == 0)
+        hipLaunchKernelGGL(pulse_kernel<float>, grid, block, 0, s, (float2 *)g.a, (float2 *)g.b, n, kind);
+    else
+        hipLaunchKernelGGL(pulse_kernel<double>, grid, block, 0, s, (double2 *)g.a, (double2 *)g.b, n, kind);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------- density projections
+// Each workgroup owns 64/N planes at fixed z and loops over the trajectories,
+// accumulating the vacuum-subtracted densities after halfPiRotate(alpha)
+// (wigner_kernel.cu:283-286,344-360) in FP64; deterministic order.
+template <typename T, int N>
+__global__ void __launch_bounds__(64) cell_sums_kernel(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b,
+                                                       int nz, int ntraj, double ca, double sa, double vac,
+                                                       double *__restrict__ out)
+{
+    constexpr int PPW = 64 / N;
+    __shared__ cpx<T> lds[PPW * N * (N + 1)];
+    const int lane = threadIdx.x, pl = lane / N, t = lane % N;
+    const int z = blockIdx.x * PPW + pl;
+    cpx<T> *tile = lds + pl * N * (N + 1);
+    const double r = 0.70710678118654752440;
+    double acc[2][N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) acc[0][x] = acc[1][x] = 0.0;
+    for (int traj = 0; traj < ntraj; ++traj) {
+        const long base = ((long)traj * nz + z) * (N * N);
+        cpx<T> v[2][N];
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            v[0][y] = a[base + y * N + t];
+            v[1][y] = b[base + y * N + t];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            dft<T, N, -1>(v[c]);
+            col2row<T, N>(tile, t, v[c]);
+            dft<T, N, -1>(v[c]);
+        }
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const double ar = v[0][x].x, ai = v[0][x].y, br = v[1][x].x, bi = v[1][x].y;
+            // a' = (a + b (sin - i cos))/sqrt2, b' = (a (-sin - i cos) + b)/sqrt2
+            const double anr = (ar + br * sa + bi * ca) * r, ani = (ai + bi * sa - br * ca) * r;
+            const double bnr = (-ar * sa + ai * ca + br) * r, bni = (-ai * sa - ar * ca + bi) * r;
+            acc[0][x] += anr * anr + ani * ani - vac;
+            acc[1][x] += bnr * bnr + bni * bni - vac;
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+        const long cell = ((long)z * N + t) * N + x;
+        out[cell * 2] = acc[0][x];
+        out[cell * 2 + 1] = acc[1][x];
+    }
+}
+
+hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, double *out, hipStream_t s)
+{
+    double ca = cos(alpha), sa = sin(alpha);
+    return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
+        using T = typename decltype(Tt)::type;
+        constexpr int N = decltype(Nt)::value;
+        constexpr int PPW = 64 / N;
+        if (g.nz % PPW) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((cell_sums_kernel<T, N>), dim3(g.nz / PPW), dim3(64), 0, s, (const cpx<T> *)g.a,
+                           (const cpx<T> *)g.b, g.nz, g.ntraj, ca, sa, vac, out);
+        return hipGetLastError();
+    });
+}
+
+// ------------------------------------------------------------- conversion
+template <typename A, typename B>
+__global__ void convert_kernel(const A *__restrict__ in, B *__restrict__ out, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (B)in[i];
+}
+
+hipError_t run_convert(int prec_from, const void *in, int prec_to, void *out, long n_complex, hipStream_t s)
+{
+    long n = 2 * n_complex;
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (prec_from == 1 && prec_to == 0)
+        hipLaunchKernelGGL((convert_kernel<double, float>), grid, block, 0, s, (const double *)in, (float *)out, n);
+    else if (prec_from == 0 && prec_to == 1)
+        hipLaunchKernelGGL((convert_kernel<float, double>), grid, block, 0, s, (const float *)in, (double *)out, n);
+    else
+        return hipMemcpyAsync(out, in, n * (prec_to == 0 ? 4 : 8), hipMemcpyDeviceToDevice, s);
+    return hipGetLastError();
+}
+
+} // namespace bec

@@ -1,0 +1,382 @@
+// The two streaming passes of one split-step, as kernel templates.
+//
+// State layout in HBM: two arrays a, b of interleaved complex, index
+// traj*cells + z*(NX*NY) + y*NX + x.  Between steps the state is held in a
+// "hybrid" representation H(z, ky, kx): transformed to k-space along x and y
+// only.  One real-time step is then exactly two passes over HBM:
+//
+//   z-pass : H --F_z(+)--> psi_k --* exp(i c dt k^2/2)/NZ--> --F_z(-)--> H
+//   xy-pass: H --F_xy(-)--> psi_x --nonlinear/loss--> --F_xy(+)/(NX NY)--> H
+//
+// which is propagate() of programs/bec_wigner_3d.cuda/wigner.cu:178-199 with
+// the trailing kinetic half-step of one step merged with the leading one of
+// the next (c = 1) and the 1/cells normalisation of wigner.cu:193 folded in.
+#pragma once
+
+#include "common.hpp"
+#include "fft_reg.hpp"
+
+namespace bec {
+
+// ============================================================ xy-plane pass
+// One wavefront (64 lanes) holds 64/N planes of N x N cells.  Lane t of a
+// plane first owns column x = t (coalesced loads), transforms along y in
+// registers, transposes through LDS to own row y = t, transforms along x,
+// applies the pointwise operator, and undoes the path.
+
+template <typename T, int N>
+__device__ __forceinline__ void col2row(cpx<T> *tile, int t, cpx<T> (&v)[N])
+{
+#pragma unroll
+    for (int y = 0; y < N; ++y) tile[y * (N + 1) + t] = v[y];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < N; ++x) v[x] = tile[t * (N + 1) + x];
+    __syncthreads();
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void row2col(cpx<T> *tile, int t, cpx<T> (&v)[N])
+{
+#pragma unroll
+    for (int x = 0; x < N; ++x) tile[t * (N + 1) + x] = v[x];
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < N; ++y) v[y] = tile[y * (N + 1) + t];
+    __syncthreads();
+}
+
+struct XyGeom {
+    int nz;          // planes per trajectory
+    long n_planes;   // planes in the launch
+};
+
+// Op interface:
+//   static constexpr int NC;         components carried through the pass
+//   static constexpr int SIGN_IN;    transform on the way in (0: none)
+//   static constexpr int SIGN_OUT;   transform on the way out (0: none)
+//   static constexpr bool WRITE;     write the components back
+//   static constexpr int NSUM;       per-WG FP64 partial sums produced
+//   const cpx<T>* src(int c); cpx<T>* dst(int c);
+//   void apply(cpx<T> (&rows)[NC][N], int y, int z, int traj, long row_index, double* sums)
+template <typename T, int N, class Op>
+__global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial)
+{
+    constexpr int PPW = 64 / N;
+    constexpr int NC = Op::NC;
+    __shared__ cpx<T> lds[PPW * N * (N + 1)];
+    const int lane = threadIdx.x;
+    const int pl = lane / N, t = lane % N;
+    const long plane = (long)blockIdx.x * PPW + pl;
+    const int z = (int)(plane % g.nz);
+    const int traj = (int)(plane / g.nz);
+    if (op.skip(traj)) return;   // uniform: a workgroup never spans two trajectories
+    cpx<T> *tile = lds + pl * N * (N + 1);
+    const long base = plane * (long)(N * N);
+
+    cpx<T> v[NC][N];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const cpx<T> *s = op.src(c);
+#pragma unroll
+        for (int y = 0; y < N; ++y) v[c][y] = s[base + y * N + t];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if constexpr (Op::SIGN_IN != 0) dft<T, N, Op::SIGN_IN>(v[c]);
+        col2row<T, N>(tile, t, v[c]);
+        if constexpr (Op::SIGN_IN != 0) dft<T, N, Op::SIGN_IN>(v[c]);
+    }
+
+    double sums[Op::NSUM > 0 ? Op::NSUM : 1];
+#pragma unroll
+    for (int i = 0; i < (Op::NSUM > 0 ? Op::NSUM : 1); ++i) sums[i] = 0.0;
+    op.apply(v, t, z, traj, base + (long)t * N, sums);
+
+    if constexpr (Op::NSUM > 0) {
+#pragma unroll
+        for (int i = 0; i < Op::NSUM; ++i) {
+            double s = group_sum<64>(sums[i]);
+            if (lane == 0) partial[(long)blockIdx.x * Op::NSUM + i] = s;
+        }
+    }
+
+    if constexpr (Op::WRITE) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if constexpr (Op::SIGN_OUT != 0) dft<T, N, Op::SIGN_OUT>(v[c]);
+            row2col<T, N>(tile, t, v[c]);
+            if constexpr (Op::SIGN_OUT != 0) dft<T, N, Op::SIGN_OUT>(v[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cpx<T> *d = op.dst(c);
+#pragma unroll
+            for (int y = 0; y < N; ++y) d[base + y * N + t] = v[c][y];
+        }
+    }
+}
+
+// ------------------------------------------------------------ xy operators
+
+// Operators that never skip a trajectory.
+struct NoSkip {
+    __device__ __forceinline__ bool skip(int) const { return false; }
+};
+// Operators that skip trajectories whose done flag is set.
+struct SkipDone {
+    const int *done;
+    __device__ __forceinline__ bool skip(int traj) const { return done && done[traj]; }
+};
+
+// Fused two-component nonlinear + loss + detuning step in x-space
+// (propagateXSpaceTwoComponent, wigner_kernel.cu:184-229), followed by the
+// 1/(NX NY) share of the x->k normalisation.
+template <typename T, int N> struct OpNonlinRT : NoSkip {
+    static constexpr int NC = 2, SIGN_IN = -1, SIGN_OUT = +1, NSUM = 0;
+    static constexpr bool WRITE = true;
+    cpx<T> *a, *b;
+    const TrajParams *P;
+    int rec_shift;
+    __device__ const cpx<T> *src(int c) const { return c == 0 ? a : b; }
+    __device__ cpx<T> *dst(int c) const { return c == 0 ? a : b; }
+    __device__ __forceinline__ void apply(cpx<T> (&r)[2][N], int y, int z, int traj, long, double *) const
+    {
+        const TrajParams &p = P[traj >> rec_shift];
+        const T half_dt = (T)(0.5 * p.dt);
+        const T g11 = (T)p.g11, g12 = (T)p.g12, g22 = (T)p.g22;
+        const T l111 = (T)p.l111, l12 = (T)p.l12, l22 = (T)p.l22, det = (T)p.detuning;
+        const T scale = (T)(1.0 / (N * N));
+        const int itmax = p.itmax;
+        const T yy = (T)(-p.ymax) + (T)p.dy * (T)y;
+        const T zz = (T)(-p.zmax) + (T)p.dz * (T)z;
+        const T vyz = yy * yy + zz * zz * (T)p.inv_lambda2;
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
+            const T V = (T)0.5 * (xx * xx + vyz);
+            cpx<T> a0 = r[0][x], b0 = r[1][x];
+            cpx<T> ca = a0, cb = b0;
+            cpx<T> da = mk<T>(0, 0), db = mk<T>(0, 0);
+            for (int it = 0; it < itmax; ++it) {
+                const T na = cnorm<T>(ca), nb = cnorm<T>(cb);
+                const T pa_re = -(l111 * na * na + l12 * nb) * (T)0.5;
+                const T pa_im = V + g11 * na + g12 * nb;
+                const T pb_re = -(l22 * nb + l12 * na) * (T)0.5;
+                const T pb_im = V + g22 * nb + g12 * na - det;
+                T sa, cs, sb, cb2;
+                sincos_t(pa_im * half_dt, &sa, &cs);
+                sincos_t(pb_im * half_dt, &sb, &cb2);
+                const T ma = exp_t(pa_re * half_dt), mb = exp_t(pb_re * half_dt);
+                da = mk<T>(ma * cs, ma * sa);
+                db = mk<T>(mb * cb2, mb * sb);
+                ca = cmul<T>(a0, da);
+                cb = cmul<T>(b0, db);
+            }
+            r[0][x] = cscale<T>(cmul<T>(ca, da), scale);
+            r[1][x] = cscale<T>(cmul<T>(cb, db), scale);
+        }
+    }
+    static __device__ __forceinline__ void sincos_t(T x, T *s, T *c)
+    {
+        if constexpr (sizeof(T) == 4) sincosf(x, s, c);
+        else sincos(x, s, c);
+    }
+    static __device__ __forceinline__ T exp_t(T x)
+    {
+        if constexpr (sizeof(T) == 4) return expf(x);
+        else return exp(x);
+    }
+};
+
+// Read-only sample: x-space sums per trajectory
+// {sum|a|^2, sum|b|^2, Re sum conj(a) b, Im sum conj(a) b} in FP64.
+template <typename T, int N> struct OpSample : NoSkip {
+    static constexpr int NC = 2, SIGN_IN = -1, SIGN_OUT = 0, NSUM = 4;
+    static constexpr bool WRITE = false;
+    const cpx<T> *a, *b;
+    __device__ const cpx<T> *src(int c) const { return c == 0 ? a : b; }
+    __device__ cpx<T> *dst(int) const { return nullptr; }
+    __device__ __forceinline__ void apply(cpx<T> (&r)[2][N], int, int, int, long, double *s) const
+    {
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+            const double ar = r[0][x].x, ai = r[0][x].y, br = r[1][x].x, bi = r[1][x].y;
+            s[0] += ar * ar + ai * ai;
+            s[1] += br * br + bi * bi;
+            s[2] += ar * br + ai * bi;
+            s[3] += ar * bi - ai * br;
+        }
+    }
+};
+
+// Plain 2D transform of one component (in -> out), scaled.
+template <typename T, int N, int SIGN> struct OpFftXY : NoSkip {
+    static constexpr int NC = 1, SIGN_IN = SIGN, SIGN_OUT = 0, NSUM = 0;
+    static constexpr bool WRITE = true;
+    const cpx<T> *in;
+    cpx<T> *out;
+    T scale;
+    __device__ const cpx<T> *src(int) const { return in; }
+    __device__ cpx<T> *dst(int) const { return out; }
+    __device__ __forceinline__ void apply(cpx<T> (&r)[1][N], int, int, int, long, double *) const
+    {
+        if (scale != (T)1)
+#pragma unroll
+            for (int x = 0; x < N; ++x) r[0][x] = cscale<T>(r[0][x], scale);
+    }
+};
+
+// ============================================================= z-pencil pass
+// A workgroup owns PW consecutive pencils (x,y columns) of one trajectory and
+// all NZ = R1 * NW of their z values.  Thread (w, p) holds z = w + NW*j,
+// j < R1 (coalesced rows of PW complex values).  Forward transform: R1-point
+// DFT in registers, twiddle exp(+-2 pi i w k'/NZ), exchange through LDS so a
+// thread owns k' = w*KPT .. w*KPT+KPT-1 for all w, NW-point DFT in registers.
+// The thread then holds psi_k at kz = k' + R1*m for m < NW.  The inverse runs
+// the same path backwards.
+
+struct ZGeom {
+    int nxy;             // pencils per trajectory (NX*NY)
+    int tiles_per_traj;  // nxy / PW
+};
+
+// Op interface:
+//   static constexpr int NC;
+//   const cpx<T>* src(int c); cpx<T>* dst(int c);
+//   void kspace(int c, cpx<T> (&X)[KPT][NW], int kk0, int q, int traj)   (k-space values, kz = kk0+i + R1*m)
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op>
+__global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const cpx<T> *__restrict__ tw)
+{
+    constexpr int NW = NZ / R1;
+    constexpr int KPT = R1 / NW;
+    constexpr int NC = Op::NC;
+    static_assert(R1 % NW == 0, "R1 must be a multiple of NW");
+    __shared__ cpx<T> lds[NZ * PW];
+    const int tid = threadIdx.x;
+    const int p = tid % PW, w = tid / PW;
+    const int traj = blockIdx.x / g.tiles_per_traj;
+    if (op.skip(traj)) return;
+    const int q = (blockIdx.x % g.tiles_per_traj) * PW + p;
+    const long base = (long)traj * NZ * g.nxy + q;
+    const int kk0 = w * KPT;
+
+    cpx<T> v[NC][R1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const cpx<T> *s = op.src(c);
+#pragma unroll
+        for (int j = 0; j < R1; ++j) v[c][j] = s[base + (long)(w + NW * j) * g.nxy];
+    }
+
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        dft<T, R1, SIGN_F>(v[c]);
+#pragma unroll
+        for (int k = 1; k < R1; ++k) {
+            cpx<T> f = tw[(w * k) % NZ];
+            if (SIGN_F < 0) f.y = -f.y;
+            v[c][k] = cmul<T>(v[c][k], f);
+        }
+        if (c > 0) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < R1; ++k) lds[(k * NW + w) * PW + p] = v[c][k];
+        __syncthreads();
+        cpx<T> X[KPT][NW];
+#pragma unroll
+        for (int i = 0; i < KPT; ++i)
+#pragma unroll
+            for (int w2 = 0; w2 < NW; ++w2) X[i][w2] = lds[((kk0 + i) * NW + w2) * PW + p];
+#pragma unroll
+        for (int i = 0; i < KPT; ++i) dft<T, NW, SIGN_F>(X[i]);
+
+        op.kspace(c, X, kk0, q, traj);
+
+        if constexpr (ROUNDTRIP) {
+#pragma unroll
+            for (int i = 0; i < KPT; ++i) {
+                dft<T, NW, -SIGN_F>(X[i]);
+#pragma unroll
+                for (int w2 = 1; w2 < NW; ++w2) {
+                    cpx<T> f = tw[(w2 * (kk0 + i)) % NZ];
+                    if (SIGN_F > 0) f.y = -f.y;
+                    X[i][w2] = cmul<T>(X[i][w2], f);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                for (int w2 = 0; w2 < NW; ++w2) lds[((kk0 + i) * NW + w2) * PW + p] = X[i][w2];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R1; ++k) v[c][k] = lds[(k * NW + w) * PW + p];
+            dft<T, R1, -SIGN_F>(v[c]);
+        } else {
+            cpx<T> *d = op.dst(c);
+#pragma unroll
+            for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                for (int m = 0; m < NW; ++m) d[base + (long)(kk0 + i + R1 * m) * g.nxy] = X[i][m];
+        }
+    }
+    if constexpr (ROUNDTRIP) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cpx<T> *d = op.dst(c);
+#pragma unroll
+            for (int j = 0; j < R1; ++j) d[base + (long)(w + NW * j) * g.nxy] = v[c][j];
+        }
+    }
+}
+
+// ------------------------------------------------------------- z operators
+
+// Kinetic phase in full k-space: psi_k *= Kxy[ky,kx] * Kz[kz], the tables
+// holding exp(i c dt (kx^2+ky^2)/2) and exp(i c dt kz^2/2)/NZ
+// (propagateKSpaceRealTime, wigner_kernel.cu:137-151; separable form).
+template <typename T, int NZ, int R1, int NC_> struct OpKinetic : NoSkip {
+    static constexpr int NC = NC_;
+    static constexpr int NW = NZ / R1, KPT = R1 / NW;
+    cpx<T> *a, *b;
+    const cpx<T> *kxy;   // [records][nxy]
+    const cpx<T> *kz;    // [records][NZ]
+    int nxy;
+    int rec_shift;
+    __device__ const cpx<T> *src(int c) const { return c == 0 ? a : b; }
+    __device__ cpx<T> *dst(int c) const { return c == 0 ? a : b; }
+    __device__ __forceinline__ void kspace(int, cpx<T> (&X)[KPT][NW], int kk0, int q, int traj) const
+    {
+        const int rec = traj >> rec_shift;
+        const cpx<T> fxy = kxy[(long)rec * nxy + q];
+        const cpx<T> *kzr = kz + (long)rec * NZ;
+#pragma unroll
+        for (int i = 0; i < KPT; ++i)
+#pragma unroll
+            for (int m = 0; m < NW; ++m) {
+                const cpx<T> f = cmul<T>(fxy, kzr[kk0 + i + R1 * m]);
+                X[i][m] = cmul<T>(X[i][m], f);
+            }
+    }
+};
+
+// Pure transform along z (one component, in -> out), scaled.
+template <typename T, int NZ, int R1> struct OpScaleZ : NoSkip {
+    static constexpr int NC = 1;
+    static constexpr int NW = NZ / R1, KPT = R1 / NW;
+    const cpx<T> *in;
+    cpx<T> *out;
+    T scale;
+    __device__ const cpx<T> *src(int) const { return in; }
+    __device__ cpx<T> *dst(int) const { return out; }
+    __device__ __forceinline__ void kspace(int, cpx<T> (&X)[KPT][NW], int, int, int) const
+    {
+        if (scale != (T)1)
+#pragma unroll
+            for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                for (int m = 0; m < NW; ++m) X[i][m] = cscale<T>(X[i][m], scale);
+    }
+};
+
+} // namespace bec

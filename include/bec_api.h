@@ -1,0 +1,237 @@
+/*
+ * bec_api.h -- C ABI of the MI355X (gfx950) split-step Wigner GPE propagator.
+ *
+ * Plain C: opaque handles, int status codes (never exit()), caller-owned host
+ * buffers, device memory owned by the context.  All GPU work is issued on the
+ * stream given at creation.
+ *
+ * Reference interface replaced (paths relative to the fjarri-attic/bec tree):
+ *   programs/bec_wigner_3d.cuda/wigner.cpp:33-40   host functions of the solver
+ *   programs/bec_wigner_3d.cuda/batchfft.h:65-108  batched FFT sub-API
+ *   programs/bec_wigner_3d.cuda/defines.h:25-210   parameter structs, state
+ *
+ * Host-side complex arrays are interleaved (re, im) doubles in the reference
+ * layout: trajectory outermost, then z, y, x with x fastest
+ * (wigner_kernel.cu:99-113, defines.h:160).
+ */
+#ifndef BEC_API_H
+#define BEC_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+enum {
+    BEC_OK = 0,
+    BEC_ERR_INVALID = 1,        /* bad argument / NULL pointer */
+    BEC_ERR_SHAPE = 2,          /* unsupported grid shape or batch */
+    BEC_ERR_ALLOC = 3,          /* device or host allocation failed */
+    BEC_ERR_HIP = 4,            /* a HIP runtime call failed */
+    BEC_ERR_NOT_CONVERGED = 5,  /* ground state hit max_iter before tol */
+    BEC_ERR_NONFINITE = 6,      /* NaN/Inf detected by an FP64 norm check */
+    BEC_ERR_STATE = 7,          /* call out of order (e.g. step before init) */
+    BEC_ERR_UNSUPPORTED = 8     /* option not available in this build */
+};
+
+enum { BEC_FP32 = 0, BEC_FP64 = 1 };          /* state / FFT / nonlinear precision */
+enum { BEC_SPACE_X = 0, BEC_SPACE_K = 1 };    /* representation for get/set state */
+enum { BEC_PULSE_HALF_PI = 0, BEC_PULSE_PI = 1 };
+enum { BEC_FFT_FORWARD = -1, BEC_FFT_INVERSE = 1 };  /* = CUFFT_FORWARD / CUFFT_INVERSE */
+
+/* Ground-state methods */
+enum {
+    BEC_GS_SPLIT_STEP = 0,  /* imaginary-time split step of wigner.cu:84-175 */
+    BEC_GS_GRADIENT = 1     /* preconditioned imaginary-time gradient flow (fixed point: H psi = mu psi) */
+};
+/* Ground-state stopping rules */
+enum {
+    BEC_GS_RULE_REFERENCE = 0,  /* |(E_new - E)/E_new| < tol on the reference integral (wigner.cu:155) */
+    BEC_GS_RULE_RESIDUAL = 1    /* ||H psi - mu psi|| / (|mu| ||psi||) < tol, evaluated in FP64 */
+};
+
+/* ModelParameters, SI units (defines.h:25-54). */
+typedef struct bec_model_params {
+    double a11, a12, a22;               /* scattering lengths, Bohr radii */
+    double m;                           /* atom mass, kg */
+    double fx, fy, fz;                  /* trap frequencies, Hz (fx radial, fz axial) */
+    int N;                              /* atom number */
+    double detuning;                    /* Hz */
+    double gamma111, gamma12, gamma22;  /* loss rates: cm^6/s, cm^3/s, cm^3/s */
+    double Va, Vb;                      /* vacuum-noise switches */
+    int nvx, nvy, nvz;                  /* lattice (powers of two) */
+    int itmax;                          /* midpoint iterations */
+    double dt_steady, t_equilib, dt_evo;/* s */
+    int ne;                             /* trajectories */
+} bec_model_params;
+
+/* CalculationParameters, natural units (defines.h:57-112). */
+typedef struct bec_calc_params {
+    int N;
+    double g11, g12, g22, mu;
+    int cells;
+    double Va, Vb, V;
+    double lambda, l_rho, t_rho;
+    double detuning, l111, l12, l22;
+    int nvx, nvy, nvz;
+    double xmax, ymax, zmax, dx, dy, dz, dkx, dky, dkz;
+    int itmax;
+    double dt_steady, t_equilib, dt_evo;
+    int ne;
+} bec_calc_params;
+
+/* fillCalculationParameters defaults (wigner.cpp:43-98). */
+int bec_default_model_params(bec_model_params *out);
+/* fillDerivedParameters (wigner.cpp:119-176), evaluated in double. */
+int bec_fill_derived_params(const bec_model_params *in, bec_calc_params *out);
+
+/* ----------------------------------------------------------- context */
+typedef struct bec_ctx bec_ctx;
+
+/* initConstants + initWaveVectors + EvolutionState::init
+ * (wigner_kernel.cu:25-55,395-399; defines.h:157-184).  One grid group. */
+int bec_create(const bec_calc_params *p, int precision, void *hip_stream, bec_ctx **out);
+
+/* Several grid groups (uneven grids / batches) stepped together.  Each group
+ * has its own parameters, lattice and trajectory count. */
+int bec_create_groups(const bec_calc_params *groups, int n_groups, int precision,
+                      void *hip_stream, bec_ctx **out);
+
+/* EvolutionState::release + releaseWaveVectors (defines.h:186-205, wigner_kernel.cu:57-61). */
+int bec_destroy(bec_ctx *ctx);
+const char *bec_strerror(int status);
+int bec_synchronize(bec_ctx *ctx);
+int bec_group_count(const bec_ctx *ctx);
+
+/* ------------------------------------------------------ ground state */
+typedef struct bec_gs_options {
+    int method;         /* BEC_GS_SPLIT_STEP | BEC_GS_GRADIENT */
+    int rule;           /* BEC_GS_RULE_REFERENCE | BEC_GS_RULE_RESIDUAL */
+    double tol;         /* stopping tolerance of the rule */
+    int max_iter;       /* iteration cap (the reference has none) */
+    int check_every;    /* residual evaluation period, iterations */
+    int precision;      /* BEC_FP32 | BEC_FP64 for the ground-state FFTs */
+    double alpha;       /* gradient flow: preconditioner shift (0: mu_TF) */
+    double beta;        /* gradient flow: momentum */
+    double tau;         /* gradient flow: step */
+} bec_gs_options;
+
+typedef struct bec_gs_report {
+    int iterations;
+    int converged;
+    double E_ref;       /* reference "energy" integral / N (wigner.cu:168-169, quirk Q3) */
+    double mu_ref;      /* reference "mu" integral / N (wigner.cu:171-172) */
+    double E;           /* <psi|H|psi>/N with the mean-field energy g n^2/2 */
+    double mu;          /* chemical potential <psi|H|psi>/<psi|psi> */
+    double N;           /* dV sum |psi|^2 */
+    double residual;    /* ||H psi - mu psi|| / (|mu| ||psi||), FP64 */
+    double elapsed_ms;  /* device time of the solve (HIP events) */
+} bec_gs_report;
+
+int bec_default_gs_options(bec_gs_options *out);
+
+/* calculateSteadyState (wigner.cu:84-175) for group `group`: imaginary-time
+ * ground state of component |1>, returned in x-space (cells complex values).
+ * guess_x (x-space, may be NULL) replaces the Thomas-Fermi start (K1). */
+int bec_calculate_steady_state(bec_ctx *ctx, int group, const double *guess_x,
+                               const bec_gs_options *opt, double *out_x, bec_gs_report *rep);
+
+/* Batched ground states: `batch` independent parameter sets that share one
+ * lattice shape (nvx, nvy, nvz taken from params[0]).  guesses (x-space,
+ * batch*cells complex) may be NULL; out_x receives batch*cells values;
+ * reps receives batch reports. */
+int bec_ground_states_batched(const bec_calc_params *params, int batch, const double *guesses,
+                              const bec_gs_options *opt, void *hip_stream, double *out_x,
+                              bec_gs_report *reps);
+
+/* Residual ||H psi - mu psi||/(|mu| ||psi||) and mu of an x-space state, FP64 on the GPU. */
+int bec_state_residual(bec_ctx *ctx, int group, const double *psi_x, double *residual, double *mu);
+
+/* ---------------------------------------------------------- evolution */
+/* initEvolution (wigner.cu:202-227): ground state + Philox vacuum noise of
+ * variance 1/(2 dV) per mode (initializeEnsembles, wigner_kernel.cu:295-317),
+ * to k-space, equilibration steps (one at least when equilibrate != 0, quirk
+ * Q5), pi/2 pulse.  gs_x: cells complex values (x-space). */
+int bec_init_evolution(bec_ctx *ctx, int group, const double *gs_x, uint64_t seed, int equilibrate);
+
+/* The hot path: n_steps of propagate() (wigner.cu:178-199) with step dt
+ * (natural units; dt <= 0 selects dt_evo) on every group.  When
+ * sample_every > 0, after every sample_every-th step the per-trajectory sums
+ * {sum|a|^2, sum|b|^2, Re sum conj(a) b, Im sum conj(a) b} of the x-space
+ * state (FP64) are recorded; sums_out (may be NULL) receives
+ * [n_steps/sample_every][total trajectories][4] doubles, groups in order. */
+int bec_propagate(bec_ctx *ctx, int n_steps, double dt, int sample_every, double *sums_out);
+
+/* Same, captured once into a hipGraph of `steps_per_graph` steps and
+ * replayed (launch-overhead-free inner loop).  n_steps must be a multiple of
+ * steps_per_graph. */
+int bec_propagate_graph(bec_ctx *ctx, int n_steps, double dt, int steps_per_graph);
+
+/* Observables of the current state (calculateEvolution tail, wigner.cu:345-410). */
+typedef struct bec_obs {
+    double t;                   /* natural units */
+    double Na, Nb;              /* sums of vacuum-subtracted densities after the virtual pi/2 pulse (alpha = 0), times dV */
+    double N_total;             /* (Na + Nb) / ne: the per-step printout of wigner.cu:369 */
+    double visibility_grid;     /* getVisibility: max over alpha in {0, 0.5, .., 6} (wigner.cu:267-277) */
+    double visibility;          /* 2 |<psi1|psi2>| / (N1 + N2), vacuum-subtracted */
+    double phase;               /* arg sum conj(a) b: mean relative phase */
+    double N1, N2;              /* vacuum-subtracted populations without the virtual pulse, dV units */
+} bec_obs;
+
+int bec_observables(bec_ctx *ctx, int group, bec_obs *out);
+
+/* calculateEvolution (wigner.cu:340-411): one propagate(dt) and the observables. */
+int bec_calculate_evolution(bec_ctx *ctx, double dt, bec_obs *out_per_group);
+
+/* Per-trajectory sums of the current x-space state: [ne][4] as in bec_propagate. */
+int bec_trajectory_sums(bec_ctx *ctx, int group, double *sums);
+
+/* Density projections after the virtual pulse halfPiRotate(alpha), ensemble
+ * summed and vacuum subtracted (wigner.cu:279-303,384-410).
+ * xy: [nvy][nvx] (sum over z), zy: [nvy][nvz] (sum over x), axial: [nvz]. */
+int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy_a, double *xy_b,
+                    double *zy_a, double *zy_b, double *axial_a, double *axial_b);
+
+/* applyHalfPiPulse / applyPiPulse (wigner_kernel.cu:320-341) on every group. */
+int bec_pulse(bec_ctx *ctx, int kind);
+
+/* Checkpoint: read / write the state of a group in x- or k-space
+ * (k-space: psi_k = (1/cells) sum_x psi_x exp(+i k.x), the reference's). */
+int bec_get_state(bec_ctx *ctx, int group, int space, double *a, double *b);
+int bec_set_state(bec_ctx *ctx, int group, int space, const double *a, const double *b);
+double bec_get_time(const bec_ctx *ctx);
+int bec_set_time(bec_ctx *ctx, double t);
+
+/* Device time of each pass kernel averaged over `iters` launches, measured
+ * with HIP events on the context stream: ms[0] z-pass, ms[1] xy-pass
+ * (nonlinear), ms[2] sample pass.  Leaves the state unchanged in value
+ * class (it runs real steps). */
+int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms);
+
+/* ------------------------------------------------- batched FFT sub-API */
+/* batchfftPlan3d / batchfftExecute / batchfftDestroy (batchfft.h:83-108).
+ * Unnormalised C2C over batch contiguous nz x ny x nx arrays in device
+ * memory; sign = BEC_FFT_FORWARD (exp(-i)) or BEC_FFT_INVERSE (exp(+i)).
+ * in == out performs the transform in place. */
+typedef struct bec_fft_plan bec_fft_plan;
+int bec_fft_plan3d(bec_fft_plan **plan, int nz, int ny, int nx, int precision, int batch);
+int bec_fft_execute(bec_fft_plan *plan, const void *d_in, void *d_out, int sign, void *hip_stream);
+int bec_fft_destroy(bec_fft_plan *plan);
+
+/* ------------------------------------------------------ device memory */
+/* Small helpers so callers without a device-memory library can drive the
+ * FFT sub-API. */
+int bec_device_alloc(void **d_ptr, size_t bytes);
+int bec_device_free(void *d_ptr);
+int bec_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int bec_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int bec_device_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEC_API_H */

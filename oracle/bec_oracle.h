@@ -1,0 +1,157 @@
+/*
+ * FP64 CPU oracle for the split-step Wigner GPE propagator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (programs/bec_wigner_3d.cuda and programs/bec_wigner_2d.cpp of
+ * fjarri-attic/bec).  It is the checker that the HIP path is compared against
+ * and the timed CPU baseline.  Nothing in the product library links to it.
+ *
+ * Conventions (identical to the reference):
+ *   - cell index = z*(nx*ny) + y*nx + x, x fastest; trajectories outermost
+ *     (programs/bec_wigner_3d.cuda/wigner_kernel.cu:99-113, defines.h:160)
+ *   - FFT sign -1 == CUFFT_FORWARD (exp(-i...)) is used k->x, sign +1 ==
+ *     CUFFT_INVERSE followed by 1/cells is used x->k
+ *     (programs/bec_wigner_3d.cuda/wigner.cu:106-107,121,129-130)
+ */
+#ifndef BEC_ORACLE_H
+#define BEC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } orc_c;
+
+/* ModelParameters, SI units (programs/bec_wigner_3d.cuda/defines.h:25-54) */
+typedef struct {
+    double a11, a12, a22;      /* Bohr radii */
+    double m;                  /* kg */
+    double fx, fy, fz;         /* Hz */
+    int N;
+    double detuning;           /* Hz */
+    double gamma111, gamma12, gamma22; /* cm^6/s, cm^3/s, cm^3/s */
+    double Va, Vb;
+    int nvx, nvy, nvz;
+    int itmax;
+    double dt_steady, t_equilib, dt_evo; /* s */
+    int ne;
+} orc_model;
+
+/* CalculationParameters, natural units (programs/bec_wigner_3d.cuda/defines.h:57-112) */
+typedef struct {
+    int N;
+    double g11, g12, g22, mu;
+    int cells;
+    double Va, Vb, V;
+    double lambda, l_rho, t_rho;
+    double detuning, l111, l12, l22;
+    int nvx, nvy, nvz;
+    double xmax, ymax, zmax, dx, dy, dz, dkx, dky, dkz;
+    int itmax;
+    double dt_steady, t_equilib, dt_evo;
+    int ne;
+} orc_calc;
+
+/* fillDerivedParameters (programs/bec_wigner_3d.cuda/wigner.cpp:119-176) */
+void orc_derive(const orc_model *m, orc_calc *c);
+
+/* k^2/2 table (wigner_kernel.cu:15-55) and trap potential (wigner_kernel.cu:99-113) */
+void orc_ktable(const orc_calc *c, double *k2half);
+double orc_potential(const orc_calc *c, int cell);
+
+/* Unnormalised batched C2C FFTs.  sign = -1: exp(-2 pi i jk/n), +1: exp(+...). */
+void orc_fft1d(orc_c *data, int n, int sign);
+void orc_fft3d(orc_c *data, int nz, int ny, int nx, int batch, int sign);
+void orc_fft_axes(orc_c *data, int nz, int ny, int nx, int batch, int sign, int axes_mask);
+void orc_dft3d_bruteforce(const orc_c *in, orc_c *out, int nz, int ny, int nx, int sign);
+
+/* ---- ground state (calculateSteadyState, wigner.cu:84-175) ---- */
+typedef struct {
+    int iterations;
+    int converged;
+    double E_ref;      /* the reference's "energy" integral (quirk Q3), per particle */
+    double mu_ref;     /* the reference's "mu" integral, per particle */
+    double E;          /* true <psi|H|psi>/N */
+    double mu;         /* true <psi|H|psi>/<psi|psi> + interaction term */
+    double N;          /* dV * sum |psi|^2 */
+    double residual;   /* ||H psi - mu psi|| / (|mu| ||psi||) */
+} orc_gs_report;
+
+#define ORC_GS_RULE_REFERENCE 0   /* stop at |(E_new-E)/E_new| < tol (wigner.cu:155) */
+#define ORC_GS_RULE_RESIDUAL  1   /* stop at residual < tol, checked every check_every iterations */
+
+int orc_ground_state(const orc_calc *c, const orc_c *guess_x, int rule, double tol,
+                     int max_iter, int check_every, orc_c *out_x, orc_gs_report *rep);
+void orc_tf_state(const orc_calc *c, orc_c *out_x);
+double orc_state_integral_ref(const orc_calc *c, const orc_c *psi_x, int energy);
+void orc_energy_mu(const orc_calc *c, const orc_c *psi_x, double *E, double *mu, double *N);
+double orc_residual(const orc_calc *c, const orc_c *psi_x, double *mu_out);
+void orc_gs_iterate(const orc_calc *c, orc_c *a_k, int n_iter, double *E_ref_out);
+
+/* ---- noise + ensembles ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_noise_cell(uint64_t seed, uint32_t traj, uint32_t cell, uint32_t stream, double out[4]);
+void orc_init_ensembles(const orc_calc *c, const orc_c *gs_x, uint64_t seed, orc_c *a_x, orc_c *b_x);
+int orc_equilibration_steps(const orc_calc *c);
+void orc_to_kspace(const orc_calc *c, orc_c *data, int ne);
+void orc_to_xspace(const orc_calc *c, orc_c *data, int ne);
+void orc_init_evolution(const orc_calc *c, const orc_c *gs_x, uint64_t seed, int equilibrate,
+                        orc_c *a_k, orc_c *b_k);
+
+/* ---- real-time evolution (propagate, wigner.cu:178-199) ---- */
+void orc_propagate(const orc_calc *c, orc_c *a_k, orc_c *b_k, int ne, double dt, int n_steps);
+void orc_kinetic_rt(const orc_calc *c, const double *k2half, orc_c *a, orc_c *b, int ne, double dt);
+void orc_nonlinear_rt(const orc_calc *c, orc_c *a_x, orc_c *b_x, int ne, double dt);
+void orc_half_pi_pulse(orc_c *a, orc_c *b, long n);
+void orc_pi_pulse(orc_c *a, orc_c *b, long n);
+
+/* ---- observables, computed the reference way: rotate per cell, then sum ---- */
+/* halfPiRotate + density (wigner_kernel.cu:283-286,344-360); returns vacuum-subtracted sums */
+void orc_rotate_sums(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne, double alpha,
+                     double *N1, double *N2);
+/* per-trajectory plane sums: [ne][4] = sum|a|^2, sum|b|^2, Re sum conj(a) b, Im sum conj(a) b */
+void orc_traj_sums(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne, double *sums);
+/* per-cell ensemble sums of vacuum-subtracted densities after halfPiRotate(alpha) */
+void orc_cell_density_sums(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne,
+                           double alpha, double *dens_a, double *dens_b);
+/* getVisibility: max over alpha = 0, 0.5, ... < 2 pi of |N1-N2|/(N1+N2), with N = |sum| */
+double orc_visibility_grid(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne);
+
+/* ---- 2D CPU program (programs/bec_wigner_2d.cpp/wigner.cpp) ---- */
+typedef struct {
+    int nvx, nvy;
+    double xmax, ymax;
+    double g11, g12, g22, E, mu;
+    double Va, Vb;
+    double dtG; int ntG; int itmax;
+    double dtW; int ntW;          /* equilibration: ntW steps including the pulse step */
+    int n_meas;                   /* samples after the pulse (reference: 1) */
+    int ne;
+    double n0;
+    uint64_t seed;
+    double noise_dev;             /* reference: 1/sqrt(dx*dy) */
+    int gs_rule;                  /* 0: fixed ntG-1 steps (reference); 1: to residual gs_tol */
+    double gs_tol; int gs_max_iter;
+} orc2d_params;
+
+typedef struct {
+    double N0;
+    int gs_iterations;
+    double gs_residual;
+} orc2d_report;
+
+/* m, var: [2][n_meas]; vis: [n_meas]; gs_x, avn: nvx*nvy (may be NULL) */
+int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c *gs_x,
+              double *avn, orc2d_report *rep);
+double orc2d_residual(const orc2d_params *p, const orc_c *psi_x, double *mu_out);
+
+int orc_num_threads(void);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
