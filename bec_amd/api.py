@@ -77,6 +77,14 @@ class Obs(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Schedule(C.Structure):
+    _fields_ = [("pw", C.c_int), ("z_r1", C.c_int), ("z_split", C.c_int), ("nt", C.c_int), ("xcd", C.c_int),
+                ("ms_per_step", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 
@@ -115,6 +123,13 @@ EXPORTS = {
     "bec_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "bec_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "bec_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "bec_set_graphs": (C.c_int, [_P, C.c_int]),
+    "bec_set_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
+    "bec_get_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
+    "bec_schedule_candidates": (C.c_int, [_P, C.c_int, C.POINTER(Schedule), C.c_int, C.POINTER(C.c_int)]),
+    "bec_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(Schedule)]),
+    "bec_device_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
 }
 
 _LIB = None
@@ -311,6 +326,35 @@ class Solver:
         ms = np.zeros(3)
         check(lib().bec_time_passes(self._ctx, group, iters, _ptr(ms)), "bec_time_passes")
         return dict(z_pass=ms[0], xy_pass=ms[1], sample_pass=ms[2])
+
+    def schedule(self, group=0):
+        sc = Schedule()
+        check(lib().bec_get_schedule(self._ctx, group, C.byref(sc)), "bec_get_schedule")
+        return sc.as_dict()
+
+    def set_schedule(self, group=0, **kw):
+        sc = Schedule()
+        check(lib().bec_get_schedule(self._ctx, group, C.byref(sc)), "bec_get_schedule")
+        for k, v in kw.items():
+            setattr(sc, k, v)
+        check(lib().bec_set_schedule(self._ctx, group, C.byref(sc)), "bec_set_schedule")
+
+    def schedule_candidates(self, group=0):
+        arr = (Schedule * 64)()
+        n = C.c_int(0)
+        check(lib().bec_schedule_candidates(self._ctx, group, arr, 64, C.byref(n)), "bec_schedule_candidates")
+        return [arr[i].as_dict() for i in range(min(n.value, 64))]
+
+    def autotune(self, steps=20, group=0):
+        """Time every candidate launch schedule on this context and keep the fastest."""
+        arr = (Schedule * 64)()
+        best = Schedule()
+        n = C.c_int(0)
+        check(lib().bec_autotune(self._ctx, group, steps, arr, 64, C.byref(n), C.byref(best)), "bec_autotune")
+        return best.as_dict(), [arr[i].as_dict() for i in range(min(n.value, 64))]
+
+    def set_graphs(self, enable):
+        check(lib().bec_set_graphs(self._ctx, 1 if enable else 0), "bec_set_graphs")
 
     def synchronize(self):
         check(lib().bec_synchronize(self._ctx), "bec_synchronize")

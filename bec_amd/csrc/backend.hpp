@@ -23,7 +23,17 @@ struct DevGroup {
     void *kxy;           // kinetic tables [KT_COUNT][n_records][nx*ny] complex
     void *kz;            // [KT_COUNT][n_records][nz] complex
     void *tw;            // z twiddles exp(+2 pi i j / nz), j < nz, complex
-    int pw;              // pencils per z-pass workgroup (0: default)
+    // launch schedule of the two hot passes (chosen by the autotuner; 0 = default)
+    int pw;              // pencils per z-pass workgroup: 16, 32, 64
+    int z_r1;            // z values per thread in registers (radix of the first z stage)
+    int z_split;         // 1: one component per z-pass workgroup (grid.y = 2)
+    int nt;              // 1: non-temporal loads/stores in both passes
+    int xcd;             // 1: XCD-aware workgroup remap (needs ntraj % 8 == 0)
+};
+
+// One schedule candidate of the hot passes.
+struct Schedule {
+    int pw, z_r1, z_split, nt, xcd;
 };
 
 enum { KT_RT_HALF = 0, KT_RT_FULL = 1, KT_IM_HALF = 2, KT_COUNT = 3 };
@@ -33,6 +43,9 @@ size_t elem_size(int prec);                       // bytes of one complex value
 long xy_workgroups(const DevGroup &g, int ntraj); // partial-sum slots of an xy pass
 int xy_wg_per_traj(const DevGroup &g);
 int default_pw(int prec, int nx, int nz);
+int default_r1(int nz);
+// the schedule candidates the hot-pass launchers are instantiated for
+int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out);
 
 hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s);
 hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s);
@@ -53,6 +66,8 @@ hipError_t run_pulse(const DevGroup &g, int kind, hipStream_t s);
 // out[cell*2 + c], x-space
 hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, double *out,
                                  hipStream_t s);
+// plain streaming copy (HBM counter calibration)
+hipError_t run_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t run_convert(int prec_from, const void *in, int prec_to, void *out, long n, hipStream_t s);
 
 } // namespace bec

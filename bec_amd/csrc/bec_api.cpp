@@ -24,7 +24,14 @@ struct bec_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *d_samples = nullptr;
     size_t sample_bytes = 0;
+    // hipGraph of STEP_BLOCK full steps [K N] over all groups; kernel
+    // arguments are device pointers only, so it stays valid when dt changes
+    hipGraph_t body_graph = nullptr;
+    hipGraphExec_t body_exec = nullptr;
+    bool use_graphs = true;
 };
+
+static const int STEP_BLOCK = 32;
 
 namespace bec {
 
@@ -274,6 +281,16 @@ extern "C" int bec_create_groups(const bec_calc_params *groups, int n_groups, in
     if (!groups || n_groups <= 0 || !out) return BEC_ERR_INVALID;
     if (precision != BEC_FP32 && precision != BEC_FP64) return BEC_ERR_INVALID;
     *out = nullptr;
+    // validate every group before touching the device
+    for (int i = 0; i < n_groups; ++i) {
+        const bec_calc_params &c = groups[i];
+        if (!shape_supported(c.nvx, c.nvy, c.nvz)) {
+            set_last_error("unsupported lattice: need nvx == nvy in {16, 32}, nvz in {64, 128, 256}");
+            return BEC_ERR_SHAPE;
+        }
+        if (c.ne <= 0 || c.cells != c.nvx * c.nvy * c.nvz || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0))
+            return BEC_ERR_INVALID;
+    }
     bec_ctx *ctx = new bec_ctx();
     ctx->prec = precision;
     if (stream) {
@@ -312,6 +329,8 @@ extern "C" int bec_destroy(bec_ctx *ctx)
 {
     CHECK_CTX(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
+    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
     for (auto &g : ctx->groups) group_release(g);
     void *p = ctx->d_samples;
     dev_free(p);
@@ -364,32 +383,74 @@ static int total_traj(const bec_ctx *ctx)
     return n;
 }
 
-// One call: K/2 N (K N)^(n-1) K/2 with K/2 + sample + K/2 at sample points.
+// count full steps [K N] (merged kinetic step + nonlinear step) on every group
+static int launch_full_steps(bec_ctx *ctx, int count)
+{
+    for (int i = 0; i < count; ++i) {
+        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_FULL, ctx->stream));
+        for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+    }
+    return BEC_OK;
+}
+
+static int ensure_body_graph(bec_ctx *ctx)
+{
+    if (ctx->body_exec) return BEC_OK;
+    hipGraph_t graph = nullptr;
+    BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int st = launch_full_steps(ctx, STEP_BLOCK);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != BEC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    BEC_HIP(e);
+    e = hipGraphInstantiate(&ctx->body_exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        ctx->body_exec = nullptr;
+        BEC_HIP(e);
+    }
+    ctx->body_graph = graph;
+    return BEC_OK;
+}
+
+static int run_full_steps(bec_ctx *ctx, int count)
+{
+    if (ctx->use_graphs && count >= STEP_BLOCK) {
+        int st = ensure_body_graph(ctx);
+        if (st != BEC_OK) return st;
+        for (int i = 0; i < count / STEP_BLOCK; ++i) BEC_HIP(hipGraphLaunch(ctx->body_exec, ctx->stream));
+        count %= STEP_BLOCK;
+    }
+    return launch_full_steps(ctx, count);
+}
+
+// One call: segments of sample_every steps, each K/2 N (K N)^(L-1) K/2 and a
+// read-only sample at its end; the trailing half step of one step and the
+// leading half step of the next are merged into one K inside a segment.
 static int propagate_impl(bec_ctx *ctx, int n_steps, double dt, int sample_every, double *d_samples)
 {
-    bool fresh = true;  // next kinetic factor is a leading half step
-    int sample_idx = 0;
+    (void)dt;
     const int ntot = total_traj(ctx);
-    for (int s = 0; s < n_steps; ++s) {
-        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, fresh ? KT_RT_HALF : KT_RT_FULL, ctx->stream));
+    int pos = 0, sample_idx = 0;
+    while (pos < n_steps) {
+        const int L = sample_every > 0 ? std::min(sample_every, n_steps - pos) : n_steps - pos;
+        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
         for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
-        const bool sample = sample_every > 0 && (s + 1) % sample_every == 0;
-        const bool last = s == n_steps - 1;
-        if (sample || last) {
-            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
-            if (sample && d_samples) {
-                long off = (long)sample_idx * ntot * 4;
-                for (auto &g : ctx->groups) {
-                    int st = sample_group(ctx, g, d_samples + off);
-                    if (st != BEC_OK) return st;
-                    off += (long)g.dev.ntraj * 4;
-                }
+        int st = run_full_steps(ctx, L - 1);
+        if (st != BEC_OK) return st;
+        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+        if (sample_every > 0 && L == sample_every && d_samples) {
+            long off = (long)sample_idx * ntot * 4;
+            for (auto &g : ctx->groups) {
+                st = sample_group(ctx, g, d_samples + off);
+                if (st != BEC_OK) return st;
+                off += (long)g.dev.ntraj * 4;
             }
-            if (sample) ++sample_idx;
-            fresh = true;
-        } else {
-            fresh = false;
+            ++sample_idx;
         }
+        pos += L;
     }
     return BEC_OK;
 }
@@ -436,7 +497,10 @@ extern "C" int bec_propagate_graph(bec_ctx *ctx, int n_steps, double dt, int ste
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const bool ug = ctx->use_graphs;
+    ctx->use_graphs = false;
     st = propagate_impl(ctx, steps_per_graph, dt, 0, nullptr);
+    ctx->use_graphs = ug;
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (st != BEC_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -826,3 +890,162 @@ const bec_calc_params *ctx_group_params(bec_ctx *ctx, int group, hipStream_t *st
     return &ctx->groups[group].calc;
 }
 } // namespace bec
+
+extern "C" int bec_set_graphs(bec_ctx *ctx, int enable)
+{
+    CHECK_CTX(ctx);
+    ctx->use_graphs = enable != 0;
+    return BEC_OK;
+}
+
+extern "C" int bec_device_copy(void *d_dst, const void *d_src, size_t bytes, void *stream)
+{
+    if (!d_dst || !d_src || bytes % 16) return BEC_ERR_INVALID;
+    BEC_HIP(bec::run_copy(d_dst, d_src, bytes, (hipStream_t)stream));
+    BEC_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return BEC_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+
+extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch)
+{
+    CHECK_GROUP(ctx, group);
+    if (!sch) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    Schedule cand[64];
+    const int n = schedule_candidates(g.dev.prec, g.dev.nx, g.dev.nz, g.dev.ntraj, cand, 64);
+    bool ok = false;
+    for (int i = 0; i < n && i < 64; ++i)
+        if (cand[i].pw == sch->pw && cand[i].z_r1 == sch->z_r1 && cand[i].z_split == sch->z_split &&
+            cand[i].nt == sch->nt && cand[i].xcd == sch->xcd)
+            ok = true;
+    if (!ok) return BEC_ERR_UNSUPPORTED;
+    g.dev.pw = sch->pw;
+    g.dev.z_r1 = sch->z_r1;
+    g.dev.z_split = sch->z_split;
+    g.dev.nt = sch->nt;
+    g.dev.xcd = sch->xcd;
+    // a captured step graph holds the old launch configuration
+    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
+    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
+    ctx->body_exec = nullptr;
+    ctx->body_graph = nullptr;
+    return BEC_OK;
+}
+
+extern "C" int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch)
+{
+    CHECK_GROUP(ctx, group);
+    if (!sch) return BEC_ERR_INVALID;
+    const DevGroup &d = ctx->groups[group].dev;
+    sch->pw = d.pw ? d.pw : default_pw(d.prec, d.nx, d.nz);
+    sch->z_r1 = d.z_r1 ? d.z_r1 : default_r1(d.nz);
+    sch->z_split = d.z_split;
+    sch->nt = d.nt;
+    sch->xcd = d.xcd;
+    sch->ms_per_step = 0;
+    return BEC_OK;
+}
+
+extern "C" int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *out, int max_out, int *count)
+{
+    CHECK_GROUP(ctx, group);
+    if (!count) return BEC_ERR_INVALID;
+    const DevGroup &d = ctx->groups[group].dev;
+    Schedule cand[64];
+    const int n = schedule_candidates(d.prec, d.nx, d.nz, d.ntraj, cand, 64);
+    for (int i = 0; i < n && i < max_out && i < 64; ++i) {
+        out[i].pw = cand[i].pw;
+        out[i].z_r1 = cand[i].z_r1;
+        out[i].z_split = cand[i].z_split;
+        out[i].nt = cand[i].nt;
+        out[i].xcd = cand[i].xcd;
+        out[i].ms_per_step = 0;
+    }
+    *count = n;
+    return BEC_OK;
+}
+
+// Time every candidate schedule on this context's own lattice and batch with
+// alternating z/xy passes (the real access pattern), keep the fastest.  The
+// state is saved before and restored after.
+extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *results, int max_results,
+                            int *n_results, bec_schedule *best)
+{
+    CHECK_GROUP(ctx, group);
+    if (steps <= 0) return BEC_ERR_INVALID;
+    if (n_results) *n_results = 0;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    int st = ensure_tables(ctx, g.tables_dt > 0 ? g.tables_dt : g.calc.dt_evo);
+    if (st != BEC_OK) return st;
+    const size_t bytes = elem_size(g.dev.prec) * g.cells * g.dev.ntraj;
+    void *sa = nullptr, *sb = nullptr;
+    BEC_HIP(dev_alloc(&sa, bytes));
+    hipError_t e = dev_alloc(&sb, bytes);
+    if (e != hipSuccess) {
+        dev_free(sa);
+        BEC_HIP(e);
+    }
+    BEC_HIP(hipMemcpyAsync(sa, g.dev.a, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    BEC_HIP(hipMemcpyAsync(sb, g.dev.b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    Schedule cand[64];
+    const int n = std::min(64, schedule_candidates(g.dev.prec, g.dev.nx, g.dev.nz, g.dev.ntraj, cand, 64));
+    const DevGroup saved = g.dev;
+    double best_ms = 1e30;
+    int best_i = -1;
+    for (int i = 0; i < n && st == BEC_OK; ++i) {
+        g.dev.pw = cand[i].pw;
+        g.dev.z_r1 = cand[i].z_r1;
+        g.dev.z_split = cand[i].z_split;
+        g.dev.nt = cand[i].nt;
+        g.dev.xcd = cand[i].xcd;
+        float ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {   // first repetition warms up
+            e = hipEventRecord(ctx->ev0, ctx->stream);
+            for (int k = 0; k < steps && e == hipSuccess; ++k) {
+                e = run_z_kinetic(g.dev, KT_RT_FULL, ctx->stream);
+                if (e == hipSuccess) e = run_xy_nonlinear(g.dev, ctx->stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+            if (e != hipSuccess) break;
+        }
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            st = hip_status(e);
+            break;
+        }
+        const double per = ms / steps;
+        if (results && i < max_results) {
+            results[i].pw = cand[i].pw;
+            results[i].z_r1 = cand[i].z_r1;
+            results[i].z_split = cand[i].z_split;
+            results[i].nt = cand[i].nt;
+            results[i].xcd = cand[i].xcd;
+            results[i].ms_per_step = per;
+        }
+        if (per < best_ms) {
+            best_ms = per;
+            best_i = i;
+        }
+    }
+    g.dev = saved;
+    if (st == BEC_OK && best_i >= 0) {
+        bec_schedule b{cand[best_i].pw, cand[best_i].z_r1, cand[best_i].z_split, cand[best_i].nt,
+                       cand[best_i].xcd, best_ms};
+        st = bec_set_schedule(ctx, group, &b);
+        if (best) *best = b;
+    }
+    e = hipMemcpyAsync(g.dev.a, sa, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g.dev.b, sb, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(sa);
+    dev_free(sb);
+    if (st != BEC_OK) return st;
+    BEC_HIP(e);
+    if (n_results) *n_results = n;
+    return BEC_OK;
+}

@@ -230,16 +230,32 @@ int final_report(GsWork &w, const bec_calc_params *params, int batch, hipStream_
 
 } // namespace
 
-int gs_solve(const bec_calc_params *params, int batch, const double *guesses, const bec_gs_options *opt,
-             hipStream_t s, double *out_x, bec_gs_report *reps)
+// Argument checks done before any device call.
+int gs_validate(const bec_calc_params *params, int batch, const bec_gs_options *opt)
 {
     if (!params || batch <= 0 || !opt) return BEC_ERR_INVALID;
-    const bec_gs_options o = *opt;
+    const bec_gs_options &o = *opt;
     if (o.method != BEC_GS_SPLIT_STEP && o.method != BEC_GS_GRADIENT) return BEC_ERR_INVALID;
     if (o.rule != BEC_GS_RULE_REFERENCE && o.rule != BEC_GS_RULE_RESIDUAL) return BEC_ERR_INVALID;
     if (o.method == BEC_GS_GRADIENT && o.rule != BEC_GS_RULE_RESIDUAL) return BEC_ERR_INVALID;
     if (o.max_iter <= 0 || !(o.tol > 0)) return BEC_ERR_INVALID;
     if (o.precision != BEC_FP32 && o.precision != BEC_FP64) return BEC_ERR_INVALID;
+    if (o.method == BEC_GS_GRADIENT && !(o.tau > 0)) return BEC_ERR_INVALID;
+    for (int i = 0; i < batch; ++i) {
+        const bec_calc_params &c = params[i];
+        if (!shape_supported(c.nvx, c.nvy, c.nvz)) return BEC_ERR_SHAPE;
+        if (c.nvx != params[0].nvx || c.nvy != params[0].nvy || c.nvz != params[0].nvz) return BEC_ERR_SHAPE;
+        if (c.N <= 0 || !(c.dt_steady > 0) || !(c.dx > 0) || !(c.dy > 0) || !(c.dz > 0)) return BEC_ERR_INVALID;
+    }
+    return BEC_OK;
+}
+
+int gs_solve(const bec_calc_params *params, int batch, const double *guesses, const bec_gs_options *opt,
+             hipStream_t s, double *out_x, bec_gs_report *reps)
+{
+    int vst = gs_validate(params, batch, opt);
+    if (vst != BEC_OK) return vst;
+    const bec_gs_options o = *opt;
     GsWork w;
     int st = setup_work(w, params, batch, o.precision, s);
     if (st != BEC_OK) return st;
@@ -380,6 +396,8 @@ extern "C" int bec_ground_states_batched(const bec_calc_params *params, int batc
     bec_gs_options o;
     if (opt) o = *opt;
     else bec_default_gs_options(&o);
+    int vst = bec::gs_validate(params, batch, &o);
+    if (vst != BEC_OK) return vst;
     hipStream_t s = (hipStream_t)stream;
     bool own = false;
     if (!s) {

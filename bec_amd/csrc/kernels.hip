@@ -23,25 +23,76 @@ int default_pw(int prec, int nx, int nz)
     return pw;
 }
 
+int default_r1(int nz) { return nz == 64 ? 8 : 16; }
+
+int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out)
+{
+    int n = 0;
+    const int r1s[2] = {default_r1(nz), nz == 64 ? 16 : 32};
+    for (int pw : {32, 64})
+        for (int r1 : r1s)
+            for (int split = 0; split < 2; ++split)
+                for (int nt = 0; nt < 2; ++nt)
+                    for (int xcd = 0; xcd < 2; ++xcd) {
+                        if ((long)nz * pw * (long)elem_size(prec) > 65536) continue;
+                        if ((nz / r1) * pw > 1024) continue;
+                        if (prec == 1 && (r1 != default_r1(nz) || nt)) continue;   // FP64: default radix only
+                        if (xcd && ntraj % 8) continue;
+                        if (n < max_out) out[n] = Schedule{pw, r1, split, nt, xcd};
+                        ++n;
+                    }
+    return n;
+}
+
 // ------------------------------------------------------------------ passes
+
+template <typename T, int NZ, int R1, int NC>
+static OpKinetic<T, NZ, R1, NC> make_kinetic(const DevGroup &g, int variant)
+{
+    const int nxy = g.nx * g.ny;
+    OpKinetic<T, NZ, R1, NC> op;
+    op.a = (cpx<T> *)g.a;
+    op.b = (cpx<T> *)g.b;
+    op.kxy = (const cpx<T> *)g.kxy + (long)variant * g.n_records * nxy;
+    op.kz = (const cpx<T> *)g.kz + (long)variant * g.n_records * NZ;
+    op.nxy = nxy;
+    op.rec_shift = g.rec_shift;
+    return op;
+}
 
 hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s)
 {
-    int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
-    return dispatch(g, [&](auto Tt, auto Nt, auto NZt) -> hipError_t {
+    const int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
+    const int r1 = g.z_r1 ? g.z_r1 : default_r1(g.nz);
+    return dispatch(g, [&](auto Tt, auto, auto NZt) -> hipError_t {
         using T = typename decltype(Tt)::type;
         constexpr int NZ = decltype(NZt)::value;
-        constexpr int R1 = z_r1<NZ>();
-        const int nxy = g.nx * g.ny;
-        OpKinetic<T, NZ, R1, 2> op;
-        op.a = (cpx<T> *)g.a;
-        op.b = (cpx<T> *)g.b;
-        op.kxy = (const cpx<T> *)g.kxy + (long)variant * g.n_records * nxy;
-        op.kz = (const cpx<T> *)g.kz + (long)variant * g.n_records * NZ;
-        op.nxy = nxy;
-        op.rec_shift = g.rec_shift;
-        return with_pw(pw, [&](auto PWt) {
-            return launch_z<T, NZ, decltype(PWt)::value, +1, true>(g, op, g.ntraj, s);
+        return with_pw(pw, [&](auto PWt) -> hipError_t {
+            constexpr int PW = decltype(PWt)::value;
+            return with_bool(g.z_split, [&](auto SPt) -> hipError_t {
+                constexpr bool SPLIT = decltype(SPt)::value;
+                constexpr int NC = SPLIT ? 1 : 2;
+                if constexpr (sizeof(T) == 8) {   // FP64: default radix, temporal accesses
+                    constexpr int R1 = z_r1<NZ>();
+                    return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, false>(
+                        g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
+                } else {
+                    return with_bool(g.nt, [&](auto NTt) -> hipError_t {
+                        constexpr bool NT = decltype(NTt)::value;
+                        if (r1 == z_r1<NZ>()) {
+                            constexpr int R1 = z_r1<NZ>();
+                            return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT>(
+                                g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
+                        }
+                        if (r1 == z_r1_alt<NZ>()) {
+                            constexpr int R1 = z_r1_alt<NZ>();
+                            return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT>(
+                                g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
+                        }
+                        return hipErrorInvalidValue;
+                    });
+                }
+            });
         });
     });
 }
@@ -56,7 +107,9 @@ hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s)
         op.b = (cpx<T> *)g.b;
         op.P = g.params;
         op.rec_shift = g.rec_shift;
-        return launch_xy<T, N>(g, op, g.ntraj, nullptr, s);
+        return with_bool(g.nt, [&](auto NTt) -> hipError_t {
+            return launch_xy<T, N, OpNonlinRT<T, N>, decltype(NTt)::value>(g, op, g.ntraj, nullptr, s, g.xcd);
+        });
     });
 }
 
@@ -68,7 +121,7 @@ hipError_t run_xy_sample(const DevGroup &g, double *partial, hipStream_t s)
         OpSample<T, N> op;
         op.a = (const cpx<T> *)g.a;
         op.b = (const cpx<T> *)g.b;
-        return launch_xy<T, N>(g, op, g.ntraj, partial, s);
+        return launch_xy<T, N>(g, op, g.ntraj, partial, s, g.xcd);
     });
 }
 
@@ -294,6 +347,22 @@ hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, do
                            (const cpx<T> *)g.b, g.nz, g.ntraj, ca, sa, vac, out);
         return hipGetLastError();
     });
+}
+
+// ------------------------------------------------------------------- copy
+__global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t run_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    const long n = (long)(bytes / 16);
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------- conversion

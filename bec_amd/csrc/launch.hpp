@@ -13,6 +13,7 @@ template <int V> using ic = std::integral_constant<int, V>;
 
 // z-pass radix split: NZ = R1 * NW with R1 values per thread in registers
 template <int NZ> constexpr int z_r1() { return NZ == 64 ? 8 : 16; }
+template <int NZ> constexpr int z_r1_alt() { return NZ == 64 ? 16 : 32; }
 
 template <class F> inline hipError_t dispatch(const DevGroup &g, F &&f)
 {
@@ -46,32 +47,39 @@ template <class F> inline hipError_t with_pw(int pw, F &&f)
     }
 }
 
-template <typename T, int NZ, int PW, int SIGN_F, bool RT, class Op>
-inline hipError_t launch_z(const DevGroup &g, const Op &op, int ntraj, hipStream_t s)
+template <class F> inline hipError_t with_bool(int v, F &&f)
 {
-    if constexpr ((long)NZ * PW * sizeof(cpx<T>) > 65536) {
+    return v ? f(std::true_type{}) : f(std::false_type{});
+}
+
+// ncomp_grid > 1: component-split launch (grid.y = ncomp_grid, Op::NC == 1)
+template <typename T, int NZ, int PW, int SIGN_F, bool RT, class Op, int R1 = z_r1<NZ>(), bool NT = false>
+inline hipError_t launch_z(const DevGroup &g, const Op &op, int ntraj, hipStream_t s, int ncomp_grid = 1,
+                           int xcd = 0)
+{
+    if constexpr ((long)NZ * PW * sizeof(cpx<T>) > 65536 || (NZ / R1) * PW > 1024) {
         return hipErrorInvalidValue;   // LDS exchange tile above 64 KiB: not instantiated
     } else {
-    constexpr int R1 = z_r1<NZ>();
-    constexpr int NW = NZ / R1;
-    const int nxy = g.nx * g.ny;
-    if (nxy % PW) return hipErrorInvalidValue;
-    ZGeom geo{nxy, nxy / PW};
-    dim3 grid((unsigned)((long)ntraj * geo.tiles_per_traj)), block(NW * PW);
-    hipLaunchKernelGGL((z_kernel<T, NZ, R1, PW, SIGN_F, RT, Op>), grid, block, 0, s, op, geo,
-                       (const cpx<T> *)g.tw);
-    return hipGetLastError();
+        constexpr int NW = NZ / R1;
+        const int nxy = g.nx * g.ny;
+        if (nxy % PW) return hipErrorInvalidValue;
+        ZGeom geo{nxy, nxy / PW, (xcd && ntraj % 8 == 0) ? 1 : 0};
+        dim3 grid((unsigned)((long)ntraj * geo.tiles_per_traj), (unsigned)ncomp_grid), block(NW * PW);
+        hipLaunchKernelGGL((z_kernel<T, NZ, R1, PW, SIGN_F, RT, Op, NT>), grid, block, 0, s, op, geo,
+                           (const cpx<T> *)g.tw);
+        return hipGetLastError();
     }
 }
 
-template <typename T, int N, class Op>
-inline hipError_t launch_xy(const DevGroup &g, const Op &op, int ntraj, double *partial, hipStream_t s)
+template <typename T, int N, class Op, bool NT = false>
+inline hipError_t launch_xy(const DevGroup &g, const Op &op, int ntraj, double *partial, hipStream_t s,
+                            int xcd = 0)
 {
     constexpr int PPW = 64 / N;
-    XyGeom geo{g.nz, (long)ntraj * g.nz};
+    XyGeom geo{g.nz, (long)ntraj * g.nz, (xcd && ntraj % 8 == 0) ? 1 : 0};
     if (geo.n_planes % PPW) return hipErrorInvalidValue;
     dim3 grid((unsigned)(geo.n_planes / PPW)), block(64);
-    hipLaunchKernelGGL((xy_kernel<T, N, Op>), grid, block, 0, s, op, geo, partial);
+    hipLaunchKernelGGL((xy_kernel<T, N, Op, NT>), grid, block, 0, s, op, geo, partial);
     return hipGetLastError();
 }
 

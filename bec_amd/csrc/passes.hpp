@@ -18,6 +18,53 @@
 
 namespace bec {
 
+// ------------------------------------------------------------ memory access
+// NT = true issues non-temporal (streaming) loads and stores: the state is
+// touched once per pass, so it should not displace reusable lines.
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+typedef double v2d_t __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename T> __device__ __forceinline__ cpx<T> ld(const cpx<T> *p)
+{
+    if constexpr (NT) {
+        if constexpr (sizeof(T) == 4) {
+            const v2f_t r = __builtin_nontemporal_load((const v2f_t *)p);
+            return mk<T>(r.x, r.y);
+        } else {
+            const v2d_t r = __builtin_nontemporal_load((const v2d_t *)p);
+            return mk<T>(r.x, r.y);
+        }
+    } else {
+        return *p;
+    }
+}
+
+template <bool NT, typename T> __device__ __forceinline__ void st(cpx<T> *p, cpx<T> v)
+{
+    if constexpr (NT) {
+        if constexpr (sizeof(T) == 4) {
+            v2f_t r = {v.x, v.y};
+            __builtin_nontemporal_store(r, (v2f_t *)p);
+        } else {
+            v2d_t r = {v.x, v.y};
+            __builtin_nontemporal_store(r, (v2d_t *)p);
+        }
+    } else {
+        *p = v;
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  With
+// the remap every tile of trajectory `traj` runs on XCD traj % 8 in both
+// passes, so a trajectory's lines stay in one XCD's L2 between passes.
+// Requires ntraj % 8 == 0; tiles_per_traj tiles per trajectory.
+__device__ __forceinline__ long xcd_remap(long b, int tiles_per_traj)
+{
+    const long xcd = b & 7, slot = b >> 3;
+    const long tl = slot / tiles_per_traj, w = slot - tl * tiles_per_traj;
+    return (tl * 8 + xcd) * tiles_per_traj + w;
+}
+
 // ============================================================ xy-plane pass
 // One wavefront (64 lanes) holds 64/N planes of N x N cells.  Lane t of a
 // plane first owns column x = t (coalesced loads), transforms along y in
@@ -49,6 +96,7 @@ __device__ __forceinline__ void row2col(cpx<T> *tile, int t, cpx<T> (&v)[N])
 struct XyGeom {
     int nz;          // planes per trajectory
     long n_planes;   // planes in the launch
+    int xcd;         // remap workgroups so a trajectory stays on one XCD
 };
 
 // Op interface:
@@ -59,7 +107,7 @@ struct XyGeom {
 //   static constexpr int NSUM;       per-WG FP64 partial sums produced
 //   const cpx<T>* src(int c); cpx<T>* dst(int c);
 //   void apply(cpx<T> (&rows)[NC][N], int y, int z, int traj, long row_index, double* sums)
-template <typename T, int N, class Op>
+template <typename T, int N, class Op, bool NT = false>
 __global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial)
 {
     constexpr int PPW = 64 / N;
@@ -67,7 +115,8 @@ __global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial
     __shared__ cpx<T> lds[PPW * N * (N + 1)];
     const int lane = threadIdx.x;
     const int pl = lane / N, t = lane % N;
-    const long plane = (long)blockIdx.x * PPW + pl;
+    const long blk = g.xcd ? xcd_remap(blockIdx.x, g.nz / PPW) : (long)blockIdx.x;
+    const long plane = blk * PPW + pl;
     const int z = (int)(plane % g.nz);
     const int traj = (int)(plane / g.nz);
     if (op.skip(traj)) return;   // uniform: a workgroup never spans two trajectories
@@ -79,7 +128,7 @@ __global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial
     for (int c = 0; c < NC; ++c) {
         const cpx<T> *s = op.src(c);
 #pragma unroll
-        for (int y = 0; y < N; ++y) v[c][y] = s[base + y * N + t];
+        for (int y = 0; y < N; ++y) v[c][y] = ld<NT, T>(s + base + y * N + t);
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -97,7 +146,7 @@ __global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial
 #pragma unroll
         for (int i = 0; i < Op::NSUM; ++i) {
             double s = group_sum<64>(sums[i]);
-            if (lane == 0) partial[(long)blockIdx.x * Op::NSUM + i] = s;
+            if (lane == 0) partial[blk * Op::NSUM + i] = s;
         }
     }
 
@@ -112,7 +161,7 @@ __global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial
         for (int c = 0; c < NC; ++c) {
             cpx<T> *d = op.dst(c);
 #pragma unroll
-            for (int y = 0; y < N; ++y) d[base + y * N + t] = v[c][y];
+            for (int y = 0; y < N; ++y) st<NT, T>(d + base + y * N + t, v[c][y]);
         }
     }
 }
@@ -239,13 +288,15 @@ template <typename T, int N, int SIGN> struct OpFftXY : NoSkip {
 struct ZGeom {
     int nxy;             // pencils per trajectory (NX*NY)
     int tiles_per_traj;  // nxy / PW
+    int xcd;             // remap workgroups so a trajectory stays on one XCD
 };
 
 // Op interface:
 //   static constexpr int NC;
 //   const cpx<T>* src(int c); cpx<T>* dst(int c);
 //   void kspace(int c, cpx<T> (&X)[KPT][NW], int kk0, int q, int traj)   (k-space values, kz = kk0+i + R1*m)
-template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op>
+// blockIdx.y selects the first component (component-split launches of NC = 1).
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT = false>
 __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const cpx<T> *__restrict__ tw)
 {
     constexpr int NW = NZ / R1;
@@ -255,18 +306,20 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
     __shared__ cpx<T> lds[NZ * PW];
     const int tid = threadIdx.x;
     const int p = tid % PW, w = tid / PW;
-    const int traj = blockIdx.x / g.tiles_per_traj;
+    const int cb = blockIdx.y;
+    const long blk = g.xcd ? xcd_remap(blockIdx.x, g.tiles_per_traj) : (long)blockIdx.x;
+    const int traj = (int)(blk / g.tiles_per_traj);
     if (op.skip(traj)) return;
-    const int q = (blockIdx.x % g.tiles_per_traj) * PW + p;
+    const int q = (int)(blk % g.tiles_per_traj) * PW + p;
     const long base = (long)traj * NZ * g.nxy + q;
     const int kk0 = w * KPT;
 
     cpx<T> v[NC][R1];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const cpx<T> *s = op.src(c);
+        const cpx<T> *s = op.src(c + cb);
 #pragma unroll
-        for (int j = 0; j < R1; ++j) v[c][j] = s[base + (long)(w + NW * j) * g.nxy];
+        for (int j = 0; j < R1; ++j) v[c][j] = ld<NT, T>(s + base + (long)(w + NW * j) * g.nxy);
     }
 
 #pragma unroll
@@ -313,19 +366,19 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
             for (int k = 0; k < R1; ++k) v[c][k] = lds[(k * NW + w) * PW + p];
             dft<T, R1, -SIGN_F>(v[c]);
         } else {
-            cpx<T> *d = op.dst(c);
+            cpx<T> *d = op.dst(c + cb);
 #pragma unroll
             for (int i = 0; i < KPT; ++i)
 #pragma unroll
-                for (int m = 0; m < NW; ++m) d[base + (long)(kk0 + i + R1 * m) * g.nxy] = X[i][m];
+                for (int m = 0; m < NW; ++m) st<NT, T>(d + base + (long)(kk0 + i + R1 * m) * g.nxy, X[i][m]);
         }
     }
     if constexpr (ROUNDTRIP) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            cpx<T> *d = op.dst(c);
+            cpx<T> *d = op.dst(c + cb);
 #pragma unroll
-            for (int j = 0; j < R1; ++j) d[base + (long)(w + NW * j) * g.nxy] = v[c][j];
+            for (int j = 0; j < R1; ++j) st<NT, T>(d + base + (long)(w + NW * j) * g.nxy, v[c][j]);
         }
     }
 }

@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the split-step Wigner GPE propagator on one MI355X.
+
+Workload (the headline configuration): 64 trajectories of a two-component
+Rb-87 condensate on a 128x16x16 lattice, complex64 state, N = 30000, losses
+K111 = 5.4e-30 cm^6/s, K12 = 0.78e-13 cm^3/s, K22 = 1.5e-14 cm^3/s, Philox
+vacuum noise (seed 0x5EED), pi/2 pulse, dt = 40 us; ensemble sums sampled
+every 250 steps.  One "step" = one propagate() over all trajectories.
+
+Prints one JSON line: trajectory-steps/s (whole job), the HBM roofline of the
+dominant pass kernel measured with HIP events on a >= 1 GiB state (2048
+trajectories, beyond the 256 MB Infinity Cache), ground-state time to
+residual 1e-6, and the FP64 CPU split-step timed on this host as a baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ALGO_BYTES_PER_TRAJ_STEP = 2 * 32768 * 8 * 2   # read + write both components once (128x16x16, c64)
+HBM_PEAK = 8.0e12
+SEED = 0x5EED
+
+
+def config1(**kw):
+    d = dict(N=30000, nvx=16, nvy=16, nvz=128, ne=64, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
+             detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
+    d.update(kw)
+    return d
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def cpu_baseline(psi, seconds):
+    """FP64 oracle split-step (same algorithm and seeds) timed on this host's cores."""
+    from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+    ne = 16
+    c = O.derive(O.model(**config1(ne=ne)))
+    a, b = O.init_evolution(c, psi, SEED, equilibrate=True)
+    t0 = time.perf_counter()
+    a, b = O.propagate(c, a, b, c.dt_evo, 2)
+    est = (time.perf_counter() - t0) / 2
+    steps = max(2, int(seconds / max(est, 1e-6)))
+    t0 = time.perf_counter()
+    a, b = O.propagate(c, a, b, c.dt_evo, steps)
+    el = time.perf_counter() - t0
+    return {"value": ne * steps / el, "unit": "trajectory-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{ne} trajectories x {steps} steps at 128x16x16 (FP64 oracle, OpenMP {threads} threads, "
+                      f"{cpu_model()}), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--traj", type=int, default=64)
+    ap.add_argument("--sample-every", type=int, default=250)
+    ap.add_argument("--roofline-traj", type=int, default=2048)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories are independent, "
+              "see the sharding note in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import bec_amd as B
+    if B.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    # ground state on the GPU: plain reference solve and residual-targeted solve
+    c1 = B.derive(B.default_model(**config1(ne=1)))
+    with B.Solver(c1, precision=B.FP64) as g:
+        psi, rep_grad, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
+                                                    precision=B.FP64, max_iter=5000)
+        psi_ref, rep_ref, _ = g.calculate_steady_state(method=B.GS_SPLIT_STEP, rule=B.GS_RULE_REFERENCE,
+                                                       tol=1e-6, precision=B.FP64)
+        # second run of each: timings without first-call setup
+        psi, rep_grad, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
+                                                    precision=B.FP64, max_iter=5000)
+
+    # headline workload
+    c = B.derive(B.default_model(**config1(ne=a.traj)))
+    s = B.Solver(c, precision=B.FP32)
+    s.set_graphs(not a.no_graphs)
+    s.init_evolution(psi, SEED, equilibrate=True)
+    s.propagate(a.warmup, sample_every=a.sample_every)
+    s.synchronize()
+    t0 = time.perf_counter()
+    samples = s.propagate(a.steps, sample_every=a.sample_every)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    obs = s.observables()
+    value = a.traj * a.steps / el
+    ms_cfg = s.time_passes(iters=200)
+    s.close()
+
+    # roofline on a state beyond the Infinity Cache
+    cb = B.derive(B.default_model(**config1(ne=a.roofline_traj)))
+    sb = B.Solver(cb, precision=B.FP32)
+    sb.init_evolution(psi, SEED, equilibrate=False)
+    sb.propagate(64)
+    ms_big = sb.time_passes(iters=50)
+    sb.synchronize()
+    nbig = 1000
+    t0 = time.perf_counter()
+    sb.propagate(nbig)
+    sb.synchronize()
+    big_rate = a.roofline_traj * nbig / (time.perf_counter() - t0)
+    sb.close()
+    per_launch = a.roofline_traj * ALGO_BYTES_PER_TRAJ_STEP
+    kern = {"z_pass": ms_big["z_pass"], "xy_pass": ms_big["xy_pass"]}
+    dom = max(kern, key=kern.get)
+    achieved = per_launch / (kern[dom] * 1e-3)
+    traffic = load_traffic()
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": (traffic or {}).get(dom), "kernel": dom,
+            "workload": f"{a.roofline_traj} trajectories x 128x16x16 x 2 components, complex64 "
+                        f"({per_launch / 2**30:.2f} GiB read + written per launch)",
+            "ms_per_launch": kern,
+            "other_kernel_GBs": {k: per_launch / (v * 1e-3) / 1e9 for k, v in kern.items()},
+            "step_traj_steps_per_s": big_rate,
+            "step_algorithmic_frac": big_rate * ALGO_BYTES_PER_TRAJ_STEP / HBM_PEAK,
+            "passes_per_step": 2}
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(psi, a.cpu_seconds)
+
+    out = {
+        "metric": "trajectory-steps/s at 128x16x16x2 + HBM GB/s; ground-state time to 1e-6 residual",
+        "value": value, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (Philox vacuum noise on the GPU-computed ground state)",
+        "config": {"workload": "64 trajectories x 128x16x16 two-component Rb-87, complex64, N=30000, losses on, "
+                               "Philox seed 0x5EED, dt=40us, sample every 250 steps",
+                   "trajectories": a.traj, "lattice": [128, 16, 16], "sample_every": a.sample_every,
+                   "graphs": not a.no_graphs},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "hbm_GBs_config_size": {k: a.traj * ALGO_BYTES_PER_TRAJ_STEP / (v * 1e-3) / 1e9
+                                for k, v in ms_cfg.items() if k != "sample_pass"},
+        "ground_state": {
+            "time_to_residual_1e-6_ms": rep_grad["elapsed_ms"], "iterations": rep_grad["iterations"],
+            "residual": rep_grad["residual"], "method": "preconditioned imaginary-time gradient flow, FP64",
+            "reference_rule_ms": rep_ref["elapsed_ms"], "reference_rule_iterations": rep_ref["iterations"],
+            "reference_rule_residual": rep_ref["residual"]},
+        "final_observables": {k: obs[k] for k in ("t", "N_total", "visibility", "visibility_grid")},
+        "n_samples": int(samples.shape[0]),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

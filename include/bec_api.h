@@ -211,6 +211,29 @@ int bec_set_time(bec_ctx *ctx, double t);
  * class (it runs real steps). */
 int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms);
 
+/* Launch schedule of the two hot passes of a group: pencils per z-pass
+ * workgroup (pw), z values per thread in registers (z_r1, the radix of the
+ * first z stage), one component per z workgroup (z_split), non-temporal
+ * loads/stores (nt), XCD-aware workgroup placement (xcd). */
+typedef struct bec_schedule {
+    int pw, z_r1, z_split, nt, xcd;
+    double ms_per_step;   /* filled by bec_autotune */
+} bec_schedule;
+
+int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch);
+int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch);
+/* Candidate schedules for the group's lattice and precision (*count of them). */
+int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *out, int max_out, int *count);
+/* Times every candidate with `steps` alternating z/xy passes on the group's
+ * own state (saved and restored) and keeps the fastest; results receives
+ * per-candidate ms_per_step, *n_results the number of candidates. */
+int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *results, int max_results,
+                 int *n_results, bec_schedule *best);
+
+/* Enable (default) or disable the replay of hipGraph-captured step blocks
+ * inside bec_propagate (disabled: one launch per pass). */
+int bec_set_graphs(bec_ctx *ctx, int enable);
+
 /* ------------------------------------------------- batched FFT sub-API */
 /* batchfftPlan3d / batchfftExecute / batchfftDestroy (batchfft.h:83-108).
  * Unnormalised C2C over batch contiguous nz x ny x nx arrays in device
@@ -229,6 +252,9 @@ int bec_device_free(void *d_ptr);
 int bec_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
 int bec_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
 int bec_device_count(int *count);
+/* Plain streaming device-to-device copy kernel (bytes % 16 == 0); used to
+ * calibrate hardware byte counters against a known traffic. */
+int bec_device_copy(void *d_dst, const void *d_src, size_t bytes, void *hip_stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,98 @@
+"""CPU: the C-ABI library loads, exports every entry point include/bec_api.h
+declares, and rejects bad arguments before touching a device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "bec_api.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bec_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import bec_amd
+    L = bec_amd.lib()
+    names = _declared()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    from bec_amd import api
+    missing = [n for n in _declared() if n not in api.EXPORTS]
+    assert not missing, missing
+
+
+def test_oracle_is_not_linked_into_the_product():
+    import bec_amd
+    L = bec_amd.lib()
+    for sym in ("orc_propagate", "orc_fft3d", "orc_ground_state"):
+        assert not hasattr(L, sym)
+    # and the product sources never include or load the oracle
+    for d in ("bec_amd", "bec_amd/csrc"):
+        for fn in os.listdir(os.path.join(ROOT, d)):
+            if fn.endswith((".py", ".cpp", ".hip", ".hpp")):
+                with open(os.path.join(ROOT, d, fn)) as f:
+                    txt = f.read()
+                assert "oracle" not in txt.replace("the oracle", "").lower() or fn == "build.py", fn
+
+
+def test_strerror_and_status_codes():
+    import bec_amd
+    L = bec_amd.lib()
+    assert L.bec_strerror(0) == b"ok"
+    assert b"shape" in L.bec_strerror(2)
+    assert L.bec_strerror(99) == b"unknown status"
+
+
+def test_invalid_arguments_rejected_on_cpu():
+    import bec_amd
+    from bec_amd import api
+    L = bec_amd.lib()
+    # NULL pointers
+    assert L.bec_fill_derived_params(None, None) == api.ERR_INVALID
+    assert L.bec_create(None, 0, None, None) == api.ERR_INVALID
+    # non power-of-two lattice
+    m = bec_amd.default_model(nvx=12)
+    c = api.CalcParams()
+    assert L.bec_fill_derived_params(C.byref(m), C.byref(c)) == api.ERR_SHAPE
+    # lattice shapes the kernels are not built for
+    for kw in (dict(nvx=16, nvy=32), dict(nvx=64, nvy=64), dict(nvz=512), dict(nvz=32)):
+        cc = bec_amd.derive(bec_amd.default_model(**kw))
+        ctx = C.c_void_p()
+        assert L.bec_create(C.byref(cc), 0, None, C.byref(ctx)) == api.ERR_SHAPE
+        assert not ctx.value
+    cc = bec_amd.derive(bec_amd.default_model())
+    ctx = C.c_void_p()
+    assert L.bec_create(C.byref(cc), 7, None, C.byref(ctx)) == api.ERR_INVALID
+    # FFT plan shapes
+    p = C.c_void_p()
+    assert L.bec_fft_plan3d(C.byref(p), 100, 16, 16, 0, 1) == api.ERR_SHAPE
+    assert L.bec_fft_plan3d(C.byref(p), 128, 16, 16, 0, 0) == api.ERR_INVALID
+    # ground-state options
+    o = bec_amd.gs_options(method=1, rule=0)
+    reps = (api.GsReport * 1)()
+    assert L.bec_ground_states_batched(C.byref(cc), 1, None, C.byref(o), None, None, reps) == api.ERR_INVALID
+    o = bec_amd.gs_options(max_iter=0)
+    assert L.bec_ground_states_batched(C.byref(cc), 1, None, C.byref(o), None, None, reps) == api.ERR_INVALID
+    o = bec_amd.gs_options()
+    bad = bec_amd.derive(bec_amd.default_model(nvz=32))
+    assert L.bec_ground_states_batched(C.byref(bad), 1, None, C.byref(o), None, None, reps) == api.ERR_SHAPE
+    # context-less calls
+    assert L.bec_propagate(None, 1, 0.0, 0, None) == api.ERR_INVALID
+    assert L.bec_observables(None, 0, None) == api.ERR_INVALID
+    assert L.bec_autotune(None, 0, 1, None, 0, None, None) == api.ERR_INVALID
+
+
+def test_default_gs_options_are_the_reference_method():
+    import bec_amd
+    o = bec_amd.gs_options()
+    assert (o.method, o.rule, o.tol, o.check_every) == (bec_amd.GS_SPLIT_STEP, bec_amd.GS_RULE_REFERENCE, 1e-6, 1)
