@@ -79,7 +79,7 @@ class Obs(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("pw", C.c_int), ("z_r1", C.c_int), ("z_split", C.c_int), ("nt", C.c_int), ("xcd", C.c_int),
-                ("ms_per_step", C.c_double)]
+                ("xy_minw", C.c_int), ("ms_per_step", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -130,6 +130,7 @@ EXPORTS = {
     "bec_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(Schedule), C.c_int, C.POINTER(C.c_int),
                                C.POINTER(Schedule)]),
     "bec_device_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "bec_time_copy": (C.c_int, [C.c_size_t, C.c_int, _D]),
 }
 
 _LIB = None
@@ -185,6 +186,13 @@ def gs_options(**kw):
     for k, v in kw.items():
         setattr(o, k, v)
     return o
+
+
+def time_copy(nbytes, iters=10):
+    """Event-timed device copy of nbytes (read + write each nbytes); returns ms per copy."""
+    ms = C.c_double()
+    check(lib().bec_time_copy(nbytes, iters, C.byref(ms)), "bec_time_copy")
+    return ms.value
 
 
 def device_count():

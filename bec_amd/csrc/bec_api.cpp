@@ -164,7 +164,15 @@ int group_setup(Group &g, const bec_calc_params &c, int prec, int ntraj, bool al
     d.ntraj = ntraj;
     d.rec_shift = 30;
     d.n_records = 1;
+    // default schedule: the autotuned choice on MI355X for 128x16x16 batches
+    // (one component per z workgroup, streaming accesses, XCD-aware placement)
     d.pw = 0;
+    d.z_r1 = 0;
+    d.z_split = 1;
+    d.nt = prec == 0 ? 1 : 0;
+    d.xcd = ntraj % 8 == 0 ? 1 : 0;
+    d.xy_minw = 1;
+    d.itmax1 = c.itmax == 1 ? 1 : 0;
     const size_t es = elem_size(prec);
     if (alloc_state) {
         BEC_HIP(dev_alloc(&d.a, es * g.cells * ntraj));
@@ -918,7 +926,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     bool ok = false;
     for (int i = 0; i < n && i < 64; ++i)
         if (cand[i].pw == sch->pw && cand[i].z_r1 == sch->z_r1 && cand[i].z_split == sch->z_split &&
-            cand[i].nt == sch->nt && cand[i].xcd == sch->xcd)
+            cand[i].nt == sch->nt && cand[i].xcd == sch->xcd && cand[i].xy_minw == sch->xy_minw)
             ok = true;
     if (!ok) return BEC_ERR_UNSUPPORTED;
     g.dev.pw = sch->pw;
@@ -926,6 +934,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     g.dev.z_split = sch->z_split;
     g.dev.nt = sch->nt;
     g.dev.xcd = sch->xcd;
+    g.dev.xy_minw = sch->xy_minw;
     // a captured step graph holds the old launch configuration
     if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
     if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
@@ -944,6 +953,7 @@ extern "C" int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch)
     sch->z_split = d.z_split;
     sch->nt = d.nt;
     sch->xcd = d.xcd;
+    sch->xy_minw = d.xy_minw ? d.xy_minw : 1;
     sch->ms_per_step = 0;
     return BEC_OK;
 }
@@ -961,6 +971,7 @@ extern "C" int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *ou
         out[i].z_split = cand[i].z_split;
         out[i].nt = cand[i].nt;
         out[i].xcd = cand[i].xcd;
+        out[i].xy_minw = cand[i].xy_minw;
         out[i].ms_per_step = 0;
     }
     *count = n;
@@ -1001,6 +1012,7 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
         g.dev.z_split = cand[i].z_split;
         g.dev.nt = cand[i].nt;
         g.dev.xcd = cand[i].xcd;
+        g.dev.xy_minw = cand[i].xy_minw;
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {   // first repetition warms up
             e = hipEventRecord(ctx->ev0, ctx->stream);
@@ -1025,6 +1037,7 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
             results[i].z_split = cand[i].z_split;
             results[i].nt = cand[i].nt;
             results[i].xcd = cand[i].xcd;
+            results[i].xy_minw = cand[i].xy_minw;
             results[i].ms_per_step = per;
         }
         if (per < best_ms) {
@@ -1035,7 +1048,7 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
     g.dev = saved;
     if (st == BEC_OK && best_i >= 0) {
         bec_schedule b{cand[best_i].pw, cand[best_i].z_r1, cand[best_i].z_split, cand[best_i].nt,
-                       cand[best_i].xcd, best_ms};
+                       cand[best_i].xcd, cand[best_i].xy_minw, best_ms};
         st = bec_set_schedule(ctx, group, &b);
         if (best) *best = b;
     }
@@ -1047,5 +1060,31 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
     if (st != BEC_OK) return st;
     BEC_HIP(e);
     if (n_results) *n_results = n;
+    return BEC_OK;
+}
+
+extern "C" int bec_time_copy(size_t bytes, int iters, double *ms_per_copy)
+{
+    if (bytes == 0 || bytes % 16 || iters <= 0 || !ms_per_copy) return BEC_ERR_INVALID;
+    void *a = nullptr, *b = nullptr;
+    BEC_HIP(dev_alloc(&a, bytes));
+    hipError_t e = dev_alloc(&b, bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = run_copy(b, a, bytes, nullptr);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int i = 0; i < iters && e == hipSuccess; ++i) e = run_copy(b, a, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(a);
+    dev_free(b);
+    BEC_HIP(e);
+    *ms_per_copy = ms / iters;
     return BEC_OK;
 }

@@ -29,6 +29,8 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
 {
     int n = 0;
     const int r1s[2] = {default_r1(nz), nz == 64 ? 16 : 32};
+    // z-pass tiles and flags first (xy_minw = 1), then the xy occupancy hint on
+    // the best-known z configuration family (split, streaming, XCD-aware)
     for (int pw : {32, 64})
         for (int r1 : r1s)
             for (int split = 0; split < 2; ++split)
@@ -38,9 +40,17 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
                         if ((nz / r1) * pw > 1024) continue;
                         if (prec == 1 && (r1 != default_r1(nz) || nt)) continue;   // FP64: default radix only
                         if (xcd && ntraj % 8) continue;
-                        if (n < max_out) out[n] = Schedule{pw, r1, split, nt, xcd};
+                        if (n < max_out) out[n] = Schedule{pw, r1, split, nt, xcd, 1};
                         ++n;
                     }
+    if (prec == 0)
+        for (int pw : {32, 64})
+            for (int minw : {5}) {
+                if ((long)nz * pw * (long)elem_size(prec) > 65536) continue;
+                const int xcd = ntraj % 8 == 0 ? 1 : 0;
+                if (n < max_out) out[n] = Schedule{pw, default_r1(nz), 1, 1, xcd, minw};
+                ++n;
+            }
     return n;
 }
 
@@ -97,19 +107,30 @@ hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s)
     });
 }
 
+template <typename T, int N, bool IT1>
+static hipError_t launch_nonlinear(const DevGroup &g, hipStream_t s)
+{
+    OpNonlinRT<T, N, IT1> op;
+    op.a = (cpx<T> *)g.a;
+    op.b = (cpx<T> *)g.b;
+    op.P = g.params;
+    op.rec_shift = g.rec_shift;
+    return with_bool(g.nt, [&](auto NTt) -> hipError_t {
+        constexpr bool NT = decltype(NTt)::value;
+        if constexpr (sizeof(T) == 4 && IT1) {
+            if (g.xy_minw == 5) return launch_xy<T, N, OpNonlinRT<T, N, IT1>, NT, 5>(g, op, g.ntraj, nullptr, s, g.xcd);
+        }
+        return launch_xy<T, N, OpNonlinRT<T, N, IT1>, NT, 1>(g, op, g.ntraj, nullptr, s, g.xcd);
+    });
+}
+
 hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s)
 {
     return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
         using T = typename decltype(Tt)::type;
         constexpr int N = decltype(Nt)::value;
-        OpNonlinRT<T, N> op;
-        op.a = (cpx<T> *)g.a;
-        op.b = (cpx<T> *)g.b;
-        op.P = g.params;
-        op.rec_shift = g.rec_shift;
-        return with_bool(g.nt, [&](auto NTt) -> hipError_t {
-            return launch_xy<T, N, OpNonlinRT<T, N>, decltype(NTt)::value>(g, op, g.ntraj, nullptr, s, g.xcd);
-        });
+        if (g.itmax1) return launch_nonlinear<T, N, true>(g, s);
+        return launch_nonlinear<T, N, false>(g, s);
     });
 }
 

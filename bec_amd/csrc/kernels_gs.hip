@@ -87,8 +87,8 @@ template <typename T, int N> struct OpGsNonlinIm : SkipDone {
         const TrajParams &p = P[traj >> rec_shift];
         const T hdt = (T)(0.5 * p.dt_steady), g11 = (T)p.g11, scale = (T)(1.0 / (N * N));
         const int itmax = p.itmax;
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const T V = potential_t<T>(p, x, y, z);
             const cpx<T> a0 = r[0][x];
             cpx<T> a = a0;
@@ -99,7 +99,7 @@ template <typename T, int N> struct OpGsNonlinIm : SkipDone {
                 a = cscale<T>(a0, da);
             }
             r[0][x] = cscale<T>(a, da * scale);
-        }
+        });
     }
 };
 
@@ -119,8 +119,8 @@ template <typename T, int N> struct OpGsMeasureRef : SkipDone {
         const int rec = traj >> rec_shift;
         const TrajParams &p = P[rec];
         const double ezv = ez[(long)rec * nz + z];
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const double ar = r[0][x].x, ai = r[0][x].y;
             const double n = ar * ar + ai * ai;
             const cpx<T> k = kcopy[row + x];
@@ -129,7 +129,7 @@ template <typename T, int N> struct OpGsMeasureRef : SkipDone {
             s[1] += n * potential_at(p, x, y, z);
             s[2] += n * n;
             s[3] += (ar * (double)k.x - ai * (double)k.y) * e;
-        }
+        });
     }
 };
 
@@ -166,8 +166,8 @@ template <typename T, int N> struct OpGsHam : SkipDone {
     {
         const TrajParams &p = P[traj >> rec_shift];
         const T sc = (T)scale[traj], g11 = (T)p.g11;
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const cpx<T> ps = cscale<T>(psi[row + x], sc);
             const T n = cnorm<T>(ps);
             const T V = potential_t<T>(p, x, y, z);
@@ -179,7 +179,7 @@ template <typename T, int N> struct OpGsHam : SkipDone {
             s[2] += (double)ps.x * kp.x + (double)ps.y * kp.y;
             s[3] += (double)V * n;
             s[4] += (double)n * n;
-        }
+        });
     }
 };
 
@@ -196,14 +196,14 @@ template <typename T, int N> struct OpGsResid : SkipDone {
     {
         const double sc = scale[traj], m = mu[traj];
         const T inv = (T)(1.0 / (N * N));
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const cpx<T> p = psi[row + x];
             const double rr = (double)r[0][x].x - m * sc * (double)p.x;
             const double ri = (double)r[0][x].y - m * sc * (double)p.y;
             s[0] += rr * rr + ri * ri;
             r[0][x] = mk<T>((T)rr * inv, (T)ri * inv);
-        }
+        });
     }
 };
 
@@ -220,8 +220,8 @@ template <typename T, int N> struct OpGsUpdate : SkipDone {
     __device__ __forceinline__ void apply(cpx<T> (&r)[1][N], int, int, int traj, long row, double *s) const
     {
         const T sc = (T)scale[traj];
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const cpx<T> ps = cscale<T>(psi[row + x], sc);
             const cpx<T> pv = prev[row + x];
             const cpx<T> d = r[0][x];
@@ -229,7 +229,7 @@ template <typename T, int N> struct OpGsUpdate : SkipDone {
             prev[row + x] = ps;
             psi[row + x] = nw;
             s[0] += (double)nw.x * nw.x + (double)nw.y * nw.y;
-        }
+        });
     }
 };
 

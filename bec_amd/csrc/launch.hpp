@@ -71,7 +71,7 @@ inline hipError_t launch_z(const DevGroup &g, const Op &op, int ntraj, hipStream
     }
 }
 
-template <typename T, int N, class Op, bool NT = false>
+template <typename T, int N, class Op, bool NT = false, int MINW = 1>
 inline hipError_t launch_xy(const DevGroup &g, const Op &op, int ntraj, double *partial, hipStream_t s,
                             int xcd = 0)
 {
@@ -79,7 +79,7 @@ inline hipError_t launch_xy(const DevGroup &g, const Op &op, int ntraj, double *
     XyGeom geo{g.nz, (long)ntraj * g.nz, (xcd && ntraj % 8 == 0) ? 1 : 0};
     if (geo.n_planes % PPW) return hipErrorInvalidValue;
     dim3 grid((unsigned)(geo.n_planes / PPW)), block(64);
-    hipLaunchKernelGGL((xy_kernel<T, N, Op, NT>), grid, block, 0, s, op, geo, partial);
+    hipLaunchKernelGGL((xy_kernel<T, N, Op, NT, MINW>), grid, block, 0, s, op, geo, partial);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "fastmath.hpp"
 #include "fft_reg.hpp"
 
 namespace bec {
@@ -107,8 +108,10 @@ struct XyGeom {
 //   static constexpr int NSUM;       per-WG FP64 partial sums produced
 //   const cpx<T>* src(int c); cpx<T>* dst(int c);
 //   void apply(cpx<T> (&rows)[NC][N], int y, int z, int traj, long row_index, double* sums)
-template <typename T, int N, class Op, bool NT = false>
-__global__ void __launch_bounds__(64) xy_kernel(Op op, XyGeom g, double *partial)
+// MINW: minimum waves per SIMD the register allocation must allow.
+template <typename T, int N, class Op, bool NT = false, int MINW = 1>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
+xy_kernel(Op op, XyGeom g, double *partial)
 {
     constexpr int PPW = 64 / N;
     constexpr int NC = Op::NC;
@@ -180,8 +183,11 @@ struct SkipDone {
 
 // Fused two-component nonlinear + loss + detuning step in x-space
 // (propagateXSpaceTwoComponent, wigner_kernel.cu:184-229), followed by the
-// 1/(NX NY) share of the x->k normalisation.
-template <typename T, int N> struct OpNonlinRT : NoSkip {
+// 1/(NX NY) share of the x->k normalisation.  IT1 = true is the straight-line
+// form for itmax = 1 (the reference default, wigner.cpp:90): densities from
+// the input, d = exp(p dt/2), output psi0 d d.  IT1 = false runs the general
+// midpoint iteration with a runtime trip count.
+template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
     static constexpr int NC = 2, SIGN_IN = -1, SIGN_OUT = +1, NSUM = 0;
     static constexpr bool WRITE = true;
     cpx<T> *a, *b;
@@ -200,15 +206,15 @@ template <typename T, int N> struct OpNonlinRT : NoSkip {
         const T yy = (T)(-p.ymax) + (T)p.dy * (T)y;
         const T zz = (T)(-p.zmax) + (T)p.dz * (T)z;
         const T vyz = yy * yy + zz * zz * (T)p.inv_lambda2;
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
             const T V = (T)0.5 * (xx * xx + vyz);
-            cpx<T> a0 = r[0][x], b0 = r[1][x];
+            const cpx<T> a0 = r[0][x], b0 = r[1][x];
             cpx<T> ca = a0, cb = b0;
             cpx<T> da = mk<T>(0, 0), db = mk<T>(0, 0);
-            for (int it = 0; it < itmax; ++it) {
-                const T na = cnorm<T>(ca), nb = cnorm<T>(cb);
+            auto factors = [&](cpx<T> ua, cpx<T> ub) {
+                const T na = cnorm<T>(ua), nb = cnorm<T>(ub);
                 const T pa_re = -(l111 * na * na + l12 * nb) * (T)0.5;
                 const T pa_im = V + g11 * na + g12 * nb;
                 const T pb_re = -(l22 * nb + l12 * na) * (T)0.5;
@@ -221,19 +227,26 @@ template <typename T, int N> struct OpNonlinRT : NoSkip {
                 db = mk<T>(mb * cb2, mb * sb);
                 ca = cmul<T>(a0, da);
                 cb = cmul<T>(b0, db);
+            };
+            if constexpr (IT1) {
+                factors(a0, b0);
+            } else {
+                for (int it = 0; it < itmax; ++it) factors(ca, cb);
             }
             r[0][x] = cscale<T>(cmul<T>(ca, da), scale);
             r[1][x] = cscale<T>(cmul<T>(cb, db), scale);
-        }
+        });
     }
+    // FP32: compact Cody-Waite/Taylor sincos (fastmath.hpp, ~1 ulp) and the
+    // hardware exponential for the small loss exponents; FP64: libm.
     static __device__ __forceinline__ void sincos_t(T x, T *s, T *c)
     {
-        if constexpr (sizeof(T) == 4) sincosf(x, s, c);
+        if constexpr (sizeof(T) == 4) fast_sincosf(x, s, c);
         else sincos(x, s, c);
     }
     static __device__ __forceinline__ T exp_t(T x)
     {
-        if constexpr (sizeof(T) == 4) return expf(x);
+        if constexpr (sizeof(T) == 4) return __expf(x);
         else return exp(x);
     }
 };
@@ -248,14 +261,14 @@ template <typename T, int N> struct OpSample : NoSkip {
     __device__ cpx<T> *dst(int) const { return nullptr; }
     __device__ __forceinline__ void apply(cpx<T> (&r)[2][N], int, int, int, long, double *s) const
     {
-#pragma unroll
-        for (int x = 0; x < N; ++x) {
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
             const double ar = r[0][x].x, ai = r[0][x].y, br = r[1][x].x, bi = r[1][x].y;
             s[0] += ar * ar + ai * ai;
             s[1] += br * br + bi * bi;
             s[2] += ar * br + ai * bi;
             s[3] += ar * bi - ai * br;
-        }
+        });
     }
 };
 

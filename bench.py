@@ -45,11 +45,13 @@ def cpu_model():
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    """HBM bytes per launch of each pass, from the committed rocprofv3 counter summary."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic_passes_2048traj_128x16x16.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f)
+        d = json.load(f)
+    return {k: v["bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
 
 
 def cpu_baseline(psi, seconds):
@@ -83,6 +85,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories are independent, "
@@ -110,6 +113,7 @@ def main():
     s = B.Solver(c, precision=B.FP32)
     s.set_graphs(not a.no_graphs)
     s.init_evolution(psi, SEED, equilibrate=True)
+    sched = s.autotune(steps=40)[0] if not a.no_autotune else s.schedule()
     s.propagate(a.warmup, sample_every=a.sample_every)
     s.synchronize()
     t0 = time.perf_counter()
@@ -125,6 +129,7 @@ def main():
     cb = B.derive(B.default_model(**config1(ne=a.roofline_traj)))
     sb = B.Solver(cb, precision=B.FP32)
     sb.init_evolution(psi, SEED, equilibrate=False)
+    sched_big = sb.autotune(steps=8)[0] if not a.no_autotune else sb.schedule()
     sb.propagate(64)
     ms_big = sb.time_passes(iters=50)
     sb.synchronize()
@@ -134,6 +139,7 @@ def main():
     sb.synchronize()
     big_rate = a.roofline_traj * nbig / (time.perf_counter() - t0)
     sb.close()
+    copy_ms = B.time_copy(1 << 30, 10)
     per_launch = a.roofline_traj * ALGO_BYTES_PER_TRAJ_STEP
     kern = {"z_pass": ms_big["z_pass"], "xy_pass": ms_big["xy_pass"]}
     dom = max(kern, key=kern.get)
@@ -147,7 +153,8 @@ def main():
             "other_kernel_GBs": {k: per_launch / (v * 1e-3) / 1e9 for k, v in kern.items()},
             "step_traj_steps_per_s": big_rate,
             "step_algorithmic_frac": big_rate * ALGO_BYTES_PER_TRAJ_STEP / HBM_PEAK,
-            "passes_per_step": 2}
+            "passes_per_step": 2, "schedule": sched_big,
+            "copy_kernel_GBs": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
 
     cpu = None
     if not a.no_cpu:
@@ -161,7 +168,7 @@ def main():
         "config": {"workload": "64 trajectories x 128x16x16 two-component Rb-87, complex64, N=30000, losses on, "
                                "Philox seed 0x5EED, dt=40us, sample every 250 steps",
                    "trajectories": a.traj, "lattice": [128, 16, 16], "sample_every": a.sample_every,
-                   "graphs": not a.no_graphs},
+                   "graphs": not a.no_graphs, "schedule": sched},
         "roofline": roof,
         "cpu_baseline": cpu,
         "hbm_GBs_config_size": {k: a.traj * ALGO_BYTES_PER_TRAJ_STEP / (v * 1e-3) / 1e9

@@ -214,9 +214,10 @@ int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms);
 /* Launch schedule of the two hot passes of a group: pencils per z-pass
  * workgroup (pw), z values per thread in registers (z_r1, the radix of the
  * first z stage), one component per z workgroup (z_split), non-temporal
- * loads/stores (nt), XCD-aware workgroup placement (xcd). */
+ * loads/stores (nt), XCD-aware workgroup placement (xcd), minimum waves per
+ * SIMD the xy pass's register allocation must allow (xy_minw: 1 or 5). */
 typedef struct bec_schedule {
-    int pw, z_r1, z_split, nt, xcd;
+    int pw, z_r1, z_split, nt, xcd, xy_minw;
     double ms_per_step;   /* filled by bec_autotune */
 } bec_schedule;
 
@@ -255,6 +256,9 @@ int bec_device_count(int *count);
 /* Plain streaming device-to-device copy kernel (bytes % 16 == 0); used to
  * calibrate hardware byte counters against a known traffic. */
 int bec_device_copy(void *d_dst, const void *d_src, size_t bytes, void *hip_stream);
+/* Device time of the same copy kernel on `bytes` (read + write), averaged over
+ * iters launches with HIP events: the achievable-bandwidth reference. */
+int bec_time_copy(size_t bytes, int iters, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

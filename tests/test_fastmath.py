@@ -1,0 +1,61 @@
+"""CPU: the device sin/cos of the nonlinear step (bec_amd/csrc/fastmath.hpp) built for
+the host -- the same IEEE single-precision operations -- against double precision."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HARNESS = r"""
+#include <stdio.h>
+#include "fastmath.hpp"
+int main() {
+    float x; float s, c;
+    while (fread(&x, sizeof(float), 1, stdin) == 1) {
+        bec::fast_sincosf(x, &s, &c);
+        fwrite(&s, sizeof(float), 1, stdout);
+        fwrite(&c, sizeof(float), 1, stdout);
+    }
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def harness():
+    d = tempfile.mkdtemp()
+    src = os.path.join(d, "h.cpp")
+    exe = os.path.join(d, "h")
+    with open(src, "w") as f:
+        f.write(HARNESS)
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "bec_amd", "csrc"), src, "-o", exe],
+                   check=True)
+    return exe
+
+
+def _run(exe, x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = subprocess.run([exe], input=x.tobytes(), stdout=subprocess.PIPE, check=True).stdout
+    sc = np.frombuffer(out, dtype=np.float32).reshape(-1, 2)
+    return sc[:, 0], sc[:, 1]
+
+
+def test_fast_sincos_accuracy(harness):
+    r = np.random.default_rng(0)
+    x = np.concatenate([np.linspace(-8, 8, 200001), r.uniform(-200, 200, 200000), r.uniform(-1e-3, 1e-3, 1000),
+                        np.array([0.0, np.pi / 4, np.pi / 2, np.pi, 3 * np.pi / 2, 1e4])]).astype(np.float32)
+    s, c = _run(harness, x)
+    xd = x.astype(np.float64)
+    es = np.abs(s - np.sin(xd))
+    ec = np.abs(c - np.cos(xd))
+    ulp = np.spacing(np.float32(1.0)) / 2   # 6e-8
+    assert es.max() < 2.5 * ulp and ec.max() < 2.5 * ulp, (es.max(), ec.max())
+    # tiny arguments: relative accuracy of sin
+    small = np.abs(xd) < 1e-3
+    rel = es[small] / np.maximum(np.abs(np.sin(xd[small])), 1e-30)
+    assert rel.max() < 2 * ulp
+    # the identity sin^2 + cos^2 = 1 holds to rounding
+    assert np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1).max() < 4 * ulp

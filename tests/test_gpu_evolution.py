@@ -106,3 +106,37 @@ def test_graph_propagation_equals_stream_propagation(bec, oracle, gs):
         a2, b2 = s2.get_state(bec.SPACE_K)
     # the graph replays 10-step blocks: K/2 K/2 at each seam instead of K (rounding only)
     assert rel_l2(a2, a1) < 1e-5 and rel_l2(b2, b1) < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 2e-5)])
+def test_midpoint_iterations_match_oracle(bec, oracle, gs, prec, tol):
+    """itmax = 2 takes the general midpoint path of the fused nonlinear step."""
+    c, oc = _pair(bec, oracle, prec, ne=2, itmax=2, detuning=-41.0)
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(gs, SEED, equilibrate=False)
+        s.propagate(10)
+        ax, bx = s.get_state(bec.SPACE_X)
+    oa, ob = oracle.init_evolution(oc, gs, SEED, equilibrate=False)
+    oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, 10)
+    assert rel_l2(ax, oracle.to_xspace(oc, oa)) < tol
+    assert rel_l2(bx, oracle.to_xspace(oc, ob)) < tol
+
+
+def test_autotune_keeps_state_and_results(bec, oracle, gs):
+    """The autotuner restores the state; every schedule computes the same step."""
+    c, _ = _pair(bec, oracle, 0, ne=8)
+    with bec.Solver(c, precision=0) as s:
+        s.init_evolution(gs, SEED)
+        a0, b0 = s.get_state(bec.SPACE_K)
+        best, res = s.autotune(steps=3)
+        a1, b1 = s.get_state(bec.SPACE_K)
+        assert np.array_equal(a0, a1) and np.array_equal(b0, b1)
+        assert len(res) >= 8 and all(r["ms_per_step"] > 0 for r in res)
+        outs = []
+        for sch in res[:: max(1, len(res) // 6)]:
+            s.set_state(a0, b0, bec.SPACE_K)
+            s.set_schedule(**{k: sch[k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw")})
+            s.propagate(5)
+            outs.append(s.get_state(bec.SPACE_K)[0])
+    for o in outs[1:]:
+        assert rel_l2(o, outs[0]) < 1e-6
