@@ -29,6 +29,10 @@ struct bec_ctx {
     hipGraph_t body_graph = nullptr;
     hipGraphExec_t body_exec = nullptr;
     bool use_graphs = true;
+    // one stream per group: independent groups run as parallel graph branches
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> aux_ev;
+    hipEvent_t fork_ev = nullptr;
 };
 
 static const int STEP_BLOCK = 32;
@@ -320,9 +324,20 @@ extern "C" int bec_create_groups(const bec_calc_params *groups, int n_groups, in
             return st;
         }
     }
-    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess) {
         bec_destroy(ctx);
         return BEC_ERR_HIP;
+    }
+    if (n_groups > 1) {
+        ctx->aux.assign(n_groups, nullptr);
+        ctx->aux_ev.assign(n_groups, nullptr);
+        for (int i = 0; i < n_groups; ++i)
+            if (hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ctx->aux_ev[i], hipEventDisableTiming) != hipSuccess) {
+                bec_destroy(ctx);
+                return BEC_ERR_HIP;
+            }
     }
     *out = ctx;
     return BEC_OK;
@@ -344,6 +359,11 @@ extern "C" int bec_destroy(bec_ctx *ctx)
     dev_free(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    for (auto e : ctx->aux_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto s : ctx->aux)
+        if (s) (void)hipStreamDestroy(s);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BEC_OK;
@@ -401,12 +421,33 @@ static int launch_full_steps(bec_ctx *ctx, int count)
     return BEC_OK;
 }
 
+// Capture STEP_BLOCK full steps.  With several groups each group's kernels go
+// on its own stream, forked from and joined back into the capture stream, so
+// the graph holds one independent branch per group (uneven grids overlap).
+static int capture_branches(bec_ctx *ctx)
+{
+    if (ctx->groups.size() == 1) return launch_full_steps(ctx, STEP_BLOCK);
+    BEC_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+    for (size_t i = 0; i < ctx->groups.size(); ++i) {
+        hipStream_t s = ctx->aux[i];
+        BEC_HIP(hipStreamWaitEvent(s, ctx->fork_ev, 0));
+        Group &g = ctx->groups[i];
+        for (int k = 0; k < STEP_BLOCK; ++k) {
+            BEC_HIP(run_z_kinetic(g.dev, KT_RT_FULL, s));
+            BEC_HIP(run_xy_nonlinear(g.dev, s));
+        }
+        BEC_HIP(hipEventRecord(ctx->aux_ev[i], s));
+    }
+    for (size_t i = 0; i < ctx->groups.size(); ++i) BEC_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_ev[i], 0));
+    return BEC_OK;
+}
+
 static int ensure_body_graph(bec_ctx *ctx)
 {
     if (ctx->body_exec) return BEC_OK;
     hipGraph_t graph = nullptr;
     BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int st = launch_full_steps(ctx, STEP_BLOCK);
+    int st = capture_branches(ctx);
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (st != BEC_OK) {
         if (graph) (void)hipGraphDestroy(graph);
