@@ -1,13 +1,13 @@
 """Precision study: FP32 state/FFT/nonlinear step with FP64 reductions against an
-all-FP64 GPU run and the FP64 CPU oracle, error stated per observable.
+all-FP64 GPU run, error stated per observable.
 
     python -m bec_amd.precision_study [--traj 128] [--steps 20000] [--nz 128 --nxy 32]
 
-Both GPU runs start from the same FP64 ground state and the same Philox seed;
+Both runs start from the same FP64 ground state and the same Philox seed;
 observables (atom numbers after the virtual pulse, visibility on the angle grid
 and analytic, mean relative phase) are sampled every --sample-every steps.
-The oracle comparison runs on --oracle-traj trajectories for --oracle-steps
-steps (CPU time).  Prints one JSON line.
+The comparison of both against the FP64 CPU oracle is tests/test_gpu_precision.py.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -67,9 +67,6 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--nxy", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0x5EED)
-    ap.add_argument("--oracle-traj", type=int, default=2)
-    ap.add_argument("--oracle-steps", type=int, default=200)
-    ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args(argv)
 
     c1 = B.derive(B.default_model(**model(a.nz, a.nxy, 1)))
@@ -86,35 +83,6 @@ def main(argv=None):
                                                       / (np.linalg.norm(a64) ** 2 + np.linalg.norm(b64) ** 2))),
            "seconds": {"fp32": t32, "fp64": t64}}
 
-    if not a.no_oracle:
-        from oracle import oracle as O
-        oc = O.derive(O.model(**model(a.nz, a.nxy, a.oracle_traj)))
-        cs = B.derive(B.default_model(**model(a.nz, a.nxy, a.oracle_traj)))
-        oa, ob = O.init_evolution(oc, gs, a.seed, equilibrate=True)
-        oa, ob = O.propagate(oc, oa, ob, oc.dt_evo, a.oracle_steps)
-        ox, oy = O.to_xspace(oc, oa), O.to_xspace(oc, ob)
-        per = {}
-        for prec, name in ((B.FP32, "fp32"), (B.FP64, "fp64")):
-            with B.Solver(cs, precision=prec) as s:
-                s.init_evolution(gs, a.seed, equilibrate=True)
-                s.propagate(a.oracle_steps)
-                ga, gb = s.get_state(B.SPACE_X)
-                obs = s.observables()
-            dV = oc.dx * oc.dy * oc.dz
-            n1, n2 = O.rotate_sums(oc, ox, oy, 0.0)
-            sums = O.traj_sums(oc, ox, oy).sum(axis=0)
-            vac = a.oracle_traj * oc.cells * oc.V / (2 * dV)
-            vis = 2 * np.hypot(sums[2], sums[3]) / (sums[0] + sums[1] - 2 * vac)
-            per[name] = {
-                "state_rel_l2_per_traj_max": float(max(
-                    np.linalg.norm(ga[i] - ox[i]) / np.linalg.norm(ox[i]) for i in range(a.oracle_traj))),
-                "Na_rel": abs(obs["Na"] - n1 * dV) / abs(n1 * dV),
-                "Nb_rel": abs(obs["Nb"] - n2 * dV) / abs(n2 * dV),
-                "visibility_grid_abs": abs(obs["visibility_grid"] - O.visibility_grid(oc, ox, oy)),
-                "visibility_abs": abs(obs["visibility"] - vis),
-                "phase_abs": abs(float(np.angle(np.exp(1j * (obs["phase"] - np.arctan2(sums[3], sums[2])))))),
-            }
-        res["vs_fp64_oracle"] = {"trajectories": a.oracle_traj, "steps": a.oracle_steps, **per}
     print(json.dumps(res))
     return res
 

@@ -1,0 +1,60 @@
+"""GPU: the FP32/FP64 split (FP32 state, FFT and nonlinear step; FP64 sums) against an
+all-FP64 GPU run and the FP64 CPU oracle, per observable, at 128x32x32."""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED
+
+
+def _model(ne, nxy=32, nz=128):
+    return dict(N=30000, nvx=nxy, nvy=nxy, nvz=nz, ne=ne, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
+                detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
+
+
+def test_precision_split_vs_fp64_oracle(bec, oracle):
+    c1 = bec.derive(bec.default_model(**_model(1)))
+    with bec.Solver(c1, precision=bec.FP64) as g:
+        gs, rep, st = g.calculate_steady_state(method=bec.GS_GRADIENT, rule=bec.GS_RULE_RESIDUAL, tol=1e-8,
+                                               precision=bec.FP64, max_iter=20000)
+    assert st == 0
+    ne, steps = 2, 200
+    oc = oracle.derive(oracle.model(**_model(ne)))
+    c = bec.derive(bec.default_model(**_model(ne)))
+    oa, ob = oracle.init_evolution(oc, gs, SEED, equilibrate=True)
+    oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, steps)
+    ox, oy = oracle.to_xspace(oc, oa), oracle.to_xspace(oc, ob)
+    dV = oc.dx * oc.dy * oc.dz
+    n1, n2 = oracle.rotate_sums(oc, ox, oy, 0.0)
+    vg = oracle.visibility_grid(oc, ox, oy)
+    S = oracle.traj_sums(oc, ox, oy).sum(axis=0)
+    phase = np.arctan2(S[3], S[2])
+    report = {}
+    for prec, tol_state, tol_obs in ((bec.FP64, 1e-11, 1e-10), (bec.FP32, 3e-5, 1e-5)):
+        with bec.Solver(c, precision=prec) as s:
+            s.init_evolution(gs, SEED, equilibrate=True)
+            s.propagate(steps)
+            ga, gb = s.get_state(bec.SPACE_X)
+            o = s.observables()
+        e = {"state": float(max(rel_l2(ga[i], ox[i]) for i in range(ne))),
+             "Na": abs(o["Na"] - n1 * dV) / abs(n1 * dV), "Nb": abs(o["Nb"] - n2 * dV) / abs(n2 * dV),
+             "visibility_grid": abs(o["visibility_grid"] - vg),
+             "phase": abs(float(np.angle(np.exp(1j * (o["phase"] - phase)))))}
+        report[prec] = e
+        assert e["state"] < tol_state, e
+        for k in ("Na", "Nb", "visibility_grid", "phase"):
+            assert e[k] < tol_obs, (prec, k, e)
+    print("precision split vs FP64 oracle (128x32x32, 2 traj, 200 steps):", report)
+
+
+def test_fp32_visibility_error_over_long_run(bec):
+    """FP32 split vs all-FP64 GPU over 4000 steps of 16 trajectories: visibility error <= 1e-4."""
+    from bec_amd import precision_study as P
+    r = P.main(["--traj", "16", "--steps", "4000", "--sample-every", "500"])
+    e = r["fp32_vs_fp64_gpu"]
+    assert e["visibility_grid"]["max_abs"] <= 1e-4, e
+    assert e["visibility"]["max_abs"] <= 1e-4, e
+    assert e["N_total"]["max_rel"] <= 1e-5, e
