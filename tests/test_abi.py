@@ -36,13 +36,13 @@ def test_oracle_is_not_linked_into_the_product():
     L = bec_amd.lib()
     for sym in ("orc_propagate", "orc_fft3d", "orc_ground_state"):
         assert not hasattr(L, sym)
-    # and the product sources never include or load the oracle
-    for d in ("bec_amd", "bec_amd/csrc"):
+    # and the product sources never import, include or load the oracle
+    pat = re.compile(r"(import\s+oracle|from\s+oracle|liboracle|bec_oracle\.h|oracle\.oracle|orc_[a-z])")
+    for d in ("bec_amd", "bec_amd/csrc", "include"):
         for fn in os.listdir(os.path.join(ROOT, d)):
-            if fn.endswith((".py", ".cpp", ".hip", ".hpp")):
+            if fn.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 with open(os.path.join(ROOT, d, fn)) as f:
-                    txt = f.read()
-                assert "oracle" not in txt.replace("the oracle", "").lower() or fn == "build.py", fn
+                    assert not pat.search(f.read()), fn
 
 
 def test_strerror_and_status_codes():
