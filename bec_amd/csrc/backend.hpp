@@ -29,13 +29,14 @@ struct DevGroup {
     int z_split;         // 1: one component per z-pass workgroup (grid.y = 2)
     int nt;              // 1: non-temporal loads/stores in both passes
     int xcd;             // 1: XCD-aware workgroup remap (needs ntraj % 8 == 0)
-    int xy_minw;         // xy pass: minimum waves per SIMD for register allocation (1, 6, 8)
+    int xy_minw;         // xy pass: minimum waves per SIMD for register allocation (1 or 5)
+    int z_half;          // 1: z-pass LDS exchange in two real-valued rounds (half the LDS)
     int itmax1;          // every record has itmax == 1: straight-line nonlinear step
 };
 
 // One schedule candidate of the hot passes.
 struct Schedule {
-    int pw, z_r1, z_split, nt, xcd, xy_minw;
+    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half;
 };
 
 enum { KT_RT_HALF = 0, KT_RT_FULL = 1, KT_IM_HALF = 2, KT_COUNT = 3 };

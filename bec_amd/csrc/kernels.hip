@@ -28,29 +28,37 @@ int default_r1(int nz) { return nz == 64 ? 8 : 16; }
 int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out)
 {
     int n = 0;
+    auto add = [&](Schedule c) {
+        if (n < max_out) out[n] = c;
+        ++n;
+    };
+    const int xcds = ntraj % 8 == 0 ? 2 : 1;
+    if (prec == 1) {   // FP64: default radix, temporal accesses
+        for (int pw : {16, 32, 64})
+            for (int split = 0; split < 2; ++split)
+                for (int half = 0; half < 2; ++half)
+                    for (int xcd = 0; xcd < xcds; ++xcd) {
+                        const long lds = (long)nz * pw * (half ? 8 : 16);
+                        if (lds > 65536 || pw > nx * nx) continue;
+                        add(Schedule{pw, default_r1(nz), split, 0, xcd, 1, half});
+                    }
+        return n;
+    }
     const int r1s[2] = {default_r1(nz), nz == 64 ? 16 : 32};
-    // z-pass tiles and flags first (xy_minw = 1), then the xy occupancy hint on
-    // the best-known z configuration family (split, streaming, XCD-aware)
     for (int pw : {32, 64})
         for (int r1 : r1s)
-            for (int split = 0; split < 2; ++split)
-                for (int nt = 0; nt < 2; ++nt)
-                    for (int xcd = 0; xcd < 2; ++xcd) {
-                        if ((long)nz * pw * (long)elem_size(prec) > 65536) continue;
-                        if ((nz / r1) * pw > 1024) continue;
-                        if (prec == 1 && (r1 != default_r1(nz) || nt)) continue;   // FP64: default radix only
-                        if (xcd && ntraj % 8) continue;
-                        if (n < max_out) out[n] = Schedule{pw, r1, split, nt, xcd, 1};
-                        ++n;
+            for (int nt = 0; nt < 2; ++nt)
+                for (int half = 0; half < 2; ++half)
+                    for (int xcd = 0; xcd < xcds; ++xcd) {
+                        const long lds = (long)nz * pw * (half ? 4 : 8);
+                        if (lds > 65536 || (nz / r1) * pw > 1024) continue;
+                        add(Schedule{pw, r1, 1, nt, xcd, 1, half});
                     }
-    if (prec == 0)
-        for (int pw : {32, 64})
-            for (int minw : {5}) {
-                if ((long)nz * pw * (long)elem_size(prec) > 65536) continue;
-                const int xcd = ntraj % 8 == 0 ? 1 : 0;
-                if (n < max_out) out[n] = Schedule{pw, default_r1(nz), 1, 1, xcd, minw};
-                ++n;
-            }
+    // the xy occupancy target on the streaming, XCD-aware family
+    for (int pw : {32, 64})
+        for (int half = 0; half < 2; ++half) add(Schedule{pw, default_r1(nz), 1, 1, xcds - 1, 5, half});
+    // the unsplit z pass for reference
+    add(Schedule{default_pw(prec, nx, nz), default_r1(nz), 0, 1, xcds - 1, 1, 0});
     return n;
 }
 
@@ -82,26 +90,34 @@ hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s)
             return with_bool(g.z_split, [&](auto SPt) -> hipError_t {
                 constexpr bool SPLIT = decltype(SPt)::value;
                 constexpr int NC = SPLIT ? 1 : 2;
-                if constexpr (sizeof(T) == 8) {   // FP64: default radix, temporal accesses
-                    constexpr int R1 = z_r1<NZ>();
-                    return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, false>(
-                        g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
-                } else {
-                    return with_bool(g.nt, [&](auto NTt) -> hipError_t {
-                        constexpr bool NT = decltype(NTt)::value;
-                        if (r1 == z_r1<NZ>()) {
-                            constexpr int R1 = z_r1<NZ>();
-                            return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT>(
-                                g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
-                        }
-                        if (r1 == z_r1_alt<NZ>()) {
-                            constexpr int R1 = z_r1_alt<NZ>();
-                            return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT>(
-                                g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
-                        }
-                        return hipErrorInvalidValue;
-                    });
-                }
+                return with_bool(g.z_half, [&](auto HFt) -> hipError_t {
+                    constexpr bool HALF = decltype(HFt)::value;
+                    if constexpr (sizeof(T) == 8) {   // FP64: default radix, temporal accesses
+                        constexpr int R1 = z_r1<NZ>();
+                        return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, false, HALF>(
+                            g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, SPLIT ? 2 : 1, g.xcd);
+                    } else if constexpr (!SPLIT) {   // unsplit: default radix, streaming, full tile
+                        constexpr int R1 = z_r1<NZ>();
+                        if (HALF) return hipErrorInvalidValue;
+                        return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, true, false>(
+                            g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, 1, g.xcd);
+                    } else {
+                        return with_bool(g.nt, [&](auto NTt) -> hipError_t {
+                            constexpr bool NT = decltype(NTt)::value;
+                            if (r1 == z_r1<NZ>()) {
+                                constexpr int R1 = z_r1<NZ>();
+                                return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT, HALF>(
+                                    g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, 2, g.xcd);
+                            }
+                            if (r1 == z_r1_alt<NZ>()) {
+                                constexpr int R1 = z_r1_alt<NZ>();
+                                return launch_z<T, NZ, PW, +1, true, OpKinetic<T, NZ, R1, NC>, R1, NT, HALF>(
+                                    g, make_kinetic<T, NZ, R1, NC>(g, variant), g.ntraj, s, 2, g.xcd);
+                            }
+                            return hipErrorInvalidValue;
+                        });
+                    }
+                });
             });
         });
     });

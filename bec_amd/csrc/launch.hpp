@@ -53,11 +53,12 @@ template <class F> inline hipError_t with_bool(int v, F &&f)
 }
 
 // ncomp_grid > 1: component-split launch (grid.y = ncomp_grid, Op::NC == 1)
-template <typename T, int NZ, int PW, int SIGN_F, bool RT, class Op, int R1 = z_r1<NZ>(), bool NT = false>
+template <typename T, int NZ, int PW, int SIGN_F, bool RT, class Op, int R1 = z_r1<NZ>(), bool NT = false,
+          bool HALF = false>
 inline hipError_t launch_z(const DevGroup &g, const Op &op, int ntraj, hipStream_t s, int ncomp_grid = 1,
                            int xcd = 0)
 {
-    if constexpr ((long)NZ * PW * sizeof(cpx<T>) > 65536 || (NZ / R1) * PW > 1024) {
+    if constexpr ((long)NZ * PW * (HALF ? sizeof(T) : sizeof(cpx<T>)) > 65536 || (NZ / R1) * PW > 1024) {
         return hipErrorInvalidValue;   // LDS exchange tile above 64 KiB: not instantiated
     } else {
         constexpr int NW = NZ / R1;
@@ -65,7 +66,7 @@ inline hipError_t launch_z(const DevGroup &g, const Op &op, int ntraj, hipStream
         if (nxy % PW) return hipErrorInvalidValue;
         ZGeom geo{nxy, nxy / PW, (xcd && ntraj % 8 == 0) ? 1 : 0};
         dim3 grid((unsigned)((long)ntraj * geo.tiles_per_traj), (unsigned)ncomp_grid), block(NW * PW);
-        hipLaunchKernelGGL((z_kernel<T, NZ, R1, PW, SIGN_F, RT, Op, NT>), grid, block, 0, s, op, geo,
+        hipLaunchKernelGGL((z_kernel<T, NZ, R1, PW, SIGN_F, RT, Op, NT, HALF>), grid, block, 0, s, op, geo,
                            (const cpx<T> *)g.tw);
         return hipGetLastError();
     }

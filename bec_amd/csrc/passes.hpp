@@ -309,16 +309,23 @@ struct ZGeom {
 //   const cpx<T>* src(int c); cpx<T>* dst(int c);
 //   void kspace(int c, cpx<T> (&X)[KPT][NW], int kk0, int q, int traj)   (k-space values, kz = kk0+i + R1*m)
 // blockIdx.y selects the first component (component-split launches of NC = 1).
-template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT = false>
+// HALF = true exchanges real and imaginary parts in two rounds through a tile
+// of NZ*PW reals (half the LDS, so more workgroups fit on a CU).
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT = false,
+          bool HALF = false>
 __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const cpx<T> *__restrict__ tw)
 {
     constexpr int NW = NZ / R1;
     constexpr int KPT = R1 / NW;
     constexpr int NC = Op::NC;
     static_assert(R1 % NW == 0, "R1 must be a multiple of NW");
-    __shared__ cpx<T> lds[NZ * PW];
+    using Slot = typename std::conditional<HALF, T, cpx<T>>::type;
+    __shared__ Slot lds[NZ * PW];
     const int tid = threadIdx.x;
-    const int p = tid % PW, w = tid / PW;
+    const int p = tid % PW;
+    // with PW = 64 a wavefront is one z subset: make w provably uniform so the
+    // twiddle and kz-table loads that depend only on it become scalar loads
+    const int w = PW == 64 ? __builtin_amdgcn_readfirstlane(tid / PW) : tid / PW;
     const int cb = blockIdx.y;
     const long blk = g.xcd ? xcd_remap(blockIdx.x, g.tiles_per_traj) : (long)blockIdx.x;
     const int traj = (int)(blk / g.tiles_per_traj);
@@ -326,6 +333,63 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
     const int q = (int)(blk % g.tiles_per_traj) * PW + p;
     const long base = (long)traj * NZ * g.nxy + q;
     const int kk0 = w * KPT;
+
+    // v[k] -> slot (k, w); X[i][w2] <- slot (kk0 + i, w2)   (forward exchange)
+    auto to_k = [&](cpx<T> (&v)[R1], cpx<T> (&X)[KPT][NW]) {
+        if constexpr (HALF) {
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+#pragma unroll
+                for (int k = 0; k < R1; ++k) lds[(k * NW + w) * PW + p] = part ? v[k].y : v[k].x;
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                    for (int w2 = 0; w2 < NW; ++w2) {
+                        const T val = lds[((kk0 + i) * NW + w2) * PW + p];
+                        if (part) X[i][w2].y = val;
+                        else X[i][w2].x = val;
+                    }
+                __syncthreads();
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R1; ++k) lds[(k * NW + w) * PW + p] = v[k];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                for (int w2 = 0; w2 < NW; ++w2) X[i][w2] = lds[((kk0 + i) * NW + w2) * PW + p];
+            __syncthreads();
+        }
+    };
+    // X[i][w2] -> slot (kk0 + i, w2); v[k] <- slot (k, w)   (inverse exchange)
+    auto from_k = [&](cpx<T> (&X)[KPT][NW], cpx<T> (&v)[R1]) {
+#pragma unroll
+        for (int part = 0; part < (HALF ? 2 : 1); ++part) {
+#pragma unroll
+            for (int i = 0; i < KPT; ++i)
+#pragma unroll
+                for (int w2 = 0; w2 < NW; ++w2) {
+                    Slot val;
+                    if constexpr (HALF) val = part ? X[i][w2].y : X[i][w2].x;
+                    else val = X[i][w2];
+                    lds[((kk0 + i) * NW + w2) * PW + p] = val;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R1; ++k) {
+                const Slot val = lds[(k * NW + w) * PW + p];
+                if constexpr (HALF) {
+                    if (part) v[k].y = val;
+                    else v[k].x = val;
+                } else {
+                    v[k] = val;
+                }
+            }
+            __syncthreads();
+        }
+    };
 
     cpx<T> v[NC][R1];
 #pragma unroll
@@ -344,15 +408,8 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
             if (SIGN_F < 0) f.y = -f.y;
             v[c][k] = cmul<T>(v[c][k], f);
         }
-        if (c > 0) __syncthreads();
-#pragma unroll
-        for (int k = 0; k < R1; ++k) lds[(k * NW + w) * PW + p] = v[c][k];
-        __syncthreads();
         cpx<T> X[KPT][NW];
-#pragma unroll
-        for (int i = 0; i < KPT; ++i)
-#pragma unroll
-            for (int w2 = 0; w2 < NW; ++w2) X[i][w2] = lds[((kk0 + i) * NW + w2) * PW + p];
+        to_k(v[c], X);
 #pragma unroll
         for (int i = 0; i < KPT; ++i) dft<T, NW, SIGN_F>(X[i]);
 
@@ -369,14 +426,7 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
                     X[i][w2] = cmul<T>(X[i][w2], f);
                 }
             }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < KPT; ++i)
-#pragma unroll
-                for (int w2 = 0; w2 < NW; ++w2) lds[((kk0 + i) * NW + w2) * PW + p] = X[i][w2];
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < R1; ++k) v[c][k] = lds[(k * NW + w) * PW + p];
+            from_k(X, v[c]);
             dft<T, R1, -SIGN_F>(v[c]);
         } else {
             cpx<T> *d = op.dst(c + cb);

@@ -215,9 +215,10 @@ int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms);
  * workgroup (pw), z values per thread in registers (z_r1, the radix of the
  * first z stage), one component per z workgroup (z_split), non-temporal
  * loads/stores (nt), XCD-aware workgroup placement (xcd), minimum waves per
- * SIMD the xy pass's register allocation must allow (xy_minw: 1 or 5). */
+ * SIMD the xy pass's register allocation must allow (xy_minw: 1 or 5), z LDS
+ * exchange in two real-valued rounds through half the LDS (z_half). */
 typedef struct bec_schedule {
-    int pw, z_r1, z_split, nt, xcd, xy_minw;
+    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half;
     double ms_per_step;   /* filled by bec_autotune */
 } bec_schedule;
 

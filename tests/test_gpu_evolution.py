@@ -135,7 +135,7 @@ def test_autotune_keeps_state_and_results(bec, oracle, gs):
         outs = []
         for sch in res[:: max(1, len(res) // 6)]:
             s.set_state(a0, b0, bec.SPACE_K)
-            s.set_schedule(**{k: sch[k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw")})
+            s.set_schedule(**{k: sch[k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half")})
             s.propagate(5)
             outs.append(s.get_state(bec.SPACE_K)[0])
     for o in outs[1:]:
