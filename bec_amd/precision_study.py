@@ -26,9 +26,9 @@ import bec_amd as B  # noqa: E402
 OBS = ("Na", "Nb", "N_total", "visibility_grid", "visibility", "phase")
 
 
-def model(nz, nxy, ne):
+def model(nz, nxy, ne, va=1.0, vb=1.0):
     return dict(N=30000, nvx=nxy, nvy=nxy, nvz=nz, ne=ne, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
-                detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
+                detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=va, Vb=vb, itmax=1)
 
 
 def run_gpu(c, gs, prec, seed, steps, every):
@@ -67,16 +67,18 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--nxy", type=int, default=32)
     ap.add_argument("--seed", type=int, default=0x5EED)
+    ap.add_argument("--va", type=float, default=1.0)
+    ap.add_argument("--vb", type=float, default=1.0)
     a = ap.parse_args(argv)
 
     c1 = B.derive(B.default_model(**model(a.nz, a.nxy, 1)))
     with B.Solver(c1, precision=B.FP64) as g:
         gs, rep, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-8,
                                               precision=B.FP64, max_iter=20000)
-    c = B.derive(B.default_model(**model(a.nz, a.nxy, a.traj)))
+    c = B.derive(B.default_model(**model(a.nz, a.nxy, a.traj, a.va, a.vb)))
     o32, t32, a32, b32 = run_gpu(c, gs, B.FP32, a.seed, a.steps, a.sample_every)
     o64, t64, a64, b64 = run_gpu(c, gs, B.FP64, a.seed, a.steps, a.sample_every)
-    res = {"lattice": [a.nz, a.nxy, a.nxy], "trajectories": a.traj, "steps": a.steps,
+    res = {"lattice": [a.nz, a.nxy, a.nxy], "trajectories": a.traj, "steps": a.steps, "Va": a.va, "Vb": a.vb,
            "sample_every": a.sample_every, "ground_state_residual": rep["residual"],
            "fp32_vs_fp64_gpu": errors(o32, o64),
            "state_rel_l2_fp32_vs_fp64": float(np.sqrt((np.linalg.norm(a32 - a64) ** 2 + np.linalg.norm(b32 - b64) ** 2)
