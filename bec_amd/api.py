@@ -68,6 +68,15 @@ class GsReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Gs2Report(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("E", C.c_double), ("mu_a", C.c_double),
+                ("mu_b", C.c_double), ("N_a", C.c_double), ("N_b", C.c_double), ("residual", C.c_double),
+                ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Obs(C.Structure):
     _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
                 ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
@@ -101,6 +110,8 @@ EXPORTS = {
     "bec_calculate_steady_state": (C.c_int, [_P, C.c_int, _P, C.POINTER(GsOptions), _P, C.POINTER(GsReport)]),
     "bec_ground_states_batched": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, C.POINTER(GsOptions), _P, _P,
                                             C.POINTER(GsReport)]),
+    "bec_ground_states_two_component": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, _P, C.POINTER(GsOptions),
+                                                  _P, _P, C.POINTER(Gs2Report)]),
     "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
     "bec_init_evolution": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int]),
     "bec_propagate": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _P]),
@@ -215,6 +226,29 @@ def ground_states_batched(params, guesses=None, **opt):
     if st not in (OK, ERR_NOT_CONVERGED):
         check(st, "bec_ground_states_batched")
     assert out.size == B * cells
+    return out, [r.as_dict() for r in reps], st
+
+
+def ground_states_two_component(params, frac_a=None, guesses=None, **opt):
+    """Two-component ground states (coupled imaginary-time GPEs, gradient flow) for a
+    list of CalcParams sharing one lattice.  Returns (out [B][2][nz][ny][nx], reports, status)."""
+    L = lib()
+    B = len(params)
+    arr = (CalcParams * B)(*params)
+    opt.setdefault("method", GS_GRADIENT)
+    opt.setdefault("rule", GS_RULE_RESIDUAL)
+    o = gs_options(**opt)
+    p0 = params[0]
+    out = np.empty((B, 2, p0.nvz, p0.nvy, p0.nvx), dtype=np.complex128)
+    reps = (Gs2Report * B)()
+    fr = None if frac_a is None else np.ascontiguousarray(np.broadcast_to(frac_a, (B,)), dtype=np.float64)
+    g = None if guesses is None else _c128(guesses)
+    if g is not None and g.size != out.size:
+        raise ValueError("guesses must hold batch * 2 * cells values")
+    st = L.bec_ground_states_two_component(arr, B, None if fr is None else _ptr(fr), None if g is None else _ptr(g),
+                                           C.byref(o), None, _ptr(out), reps)
+    if st not in (OK, ERR_NOT_CONVERGED):
+        check(st, "bec_ground_states_two_component")
     return out, [r.as_dict() for r in reps], st
 
 

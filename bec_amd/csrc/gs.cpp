@@ -24,6 +24,9 @@ struct GsWork {
     double *exy = nullptr, *ez = nullptr, *mu_tf = nullptr, *ones = nullptr;
     GsScalars sc{};
     int *zero_done = nullptr;
+    double *target = nullptr;      // pair mode: per-trajectory atom numbers
+    int pair = 0;                  // 1: trajectories 2j, 2j+1 are the components of set j
+    hipStream_t stream = nullptr;  // the solve's stream: clears are ordered with its kernels
     std::vector<void *> owned;
 
     ~GsWork()
@@ -37,7 +40,9 @@ struct GsWork {
         hipError_t e = dev_alloc(&q, bytes);
         if (e == hipSuccess) {
             owned.push_back(q);
-            hipMemset(q, 0, bytes);
+            // on the solve's (non-blocking) stream: a null-stream memset is not
+            // ordered with it and could land after the first uploads or kernels
+            e = hipMemsetAsync(q, 0, bytes, stream);
         }
         *p = (Ptr *)q;
         return e;
@@ -64,22 +69,28 @@ void put_c(std::vector<char> &buf, long idx, int prec, double re, double im)
     }
 }
 
-// Per-record parameters and tables for a batch of parameter sets.
-int setup_work(GsWork &w, const bec_calc_params *params, int batch, int prec, hipStream_t s)
+// Per-record parameters and tables for a batch of parameter sets.  One record
+// per set; with frac_a != nullptr (pair mode) each set has two trajectories,
+// its components, with atom numbers frac_a N and (1 - frac_a) N.
+int setup_work(GsWork &w, const bec_calc_params *params, int batch, int prec, hipStream_t s,
+               const double *frac_a = nullptr)
 {
     const bec_calc_params &c0 = params[0];
     for (int i = 1; i < batch; ++i)
         if (params[i].nvx != c0.nvx || params[i].nvy != c0.nvy || params[i].nvz != c0.nvz) return BEC_ERR_SHAPE;
-    int st = group_setup(w.g, c0, prec, batch, false);
+    w.stream = s;
+    w.pair = frac_a ? 1 : 0;
+    const int ntraj = batch << w.pair;
+    int st = group_setup(w.g, c0, prec, ntraj, false);
     if (st != BEC_OK) return st;
     DevGroup &d = w.g.dev;
-    // one parameter record per trajectory
+    // one parameter record per set
     void *p = d.params;
     dev_free(p);
     dev_free(d.kxy);
     dev_free(d.kz);
     d.params = nullptr;
-    d.rec_shift = 0;
+    d.rec_shift = w.pair;
     d.n_records = batch;
     const int nx = c0.nvx, ny = c0.nvy, nz = c0.nvz, nxy = nx * ny;
     const long cells = w.g.cells;
@@ -117,29 +128,40 @@ int setup_work(GsWork &w, const bec_calc_params *params, int batch, int prec, hi
     GS_HIP(hipMemcpyAsync(w.exy, exy.data(), sizeof(double) * exy.size(), hipMemcpyHostToDevice, s));
     GS_HIP(hipMemcpyAsync(w.ez, ez.data(), sizeof(double) * ez.size(), hipMemcpyHostToDevice, s));
     GS_HIP(hipMemcpyAsync(w.mu_tf, mu.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s));
-    const size_t sbytes = es * cells * batch;
+    const size_t sbytes = es * cells * ntraj;
     GS_HIP(w.alloc(&w.X, sbytes));
     GS_HIP(w.alloc(&w.W, sbytes));
     GS_HIP(w.alloc(&w.P, sbytes));
     GS_HIP(w.alloc(&w.KC, sbytes));
-    const long slots = xy_workgroups(d, batch);
+    const long slots = xy_workgroups(d, ntraj);
     GS_HIP(w.alloc(&w.ph, sizeof(double) * 5 * slots));
     GS_HIP(w.alloc(&w.pr, sizeof(double) * 1 * slots));
     GS_HIP(w.alloc(&w.pm, sizeof(double) * 4 * slots));
     GS_HIP(w.alloc(&w.pn, sizeof(double) * 1 * slots));
-    GS_HIP(w.alloc(&w.sums, sizeof(double) * 16 * batch));
-    GS_HIP(w.alloc(&w.ones, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.s, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.mu, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.e_prev, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.res, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.e_ref, sizeof(double) * batch));
-    GS_HIP(w.alloc(&w.sc.iters, sizeof(int) * batch));
-    GS_HIP(w.alloc(&w.sc.done, sizeof(int) * batch));
-    GS_HIP(w.alloc(&w.zero_done, sizeof(int) * batch));
-    std::vector<double> ones(batch, 1.0);
-    GS_HIP(hipMemcpyAsync(w.ones, ones.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s));
-    GS_HIP(hipMemcpyAsync(w.sc.s, ones.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s));
+    GS_HIP(w.alloc(&w.sums, sizeof(double) * 16 * ntraj));
+    GS_HIP(w.alloc(&w.ones, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.s, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.mu, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.e_prev, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.res, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.e_ref, sizeof(double) * ntraj));
+    GS_HIP(w.alloc(&w.sc.iters, sizeof(int) * ntraj));
+    GS_HIP(w.alloc(&w.sc.done, sizeof(int) * ntraj));
+    GS_HIP(w.alloc(&w.zero_done, sizeof(int) * ntraj));
+    w.sc.pair = w.pair;
+    w.sc.target = nullptr;
+    std::vector<double> ones(ntraj, 1.0), tgt(ntraj);
+    if (w.pair) {
+        for (int r = 0; r < batch; ++r) {
+            tgt[2 * r] = frac_a[r] * params[r].N;
+            tgt[2 * r + 1] = (1.0 - frac_a[r]) * params[r].N;
+        }
+        GS_HIP(w.alloc(&w.target, sizeof(double) * ntraj));
+        GS_HIP(hipMemcpyAsync(w.target, tgt.data(), sizeof(double) * ntraj, hipMemcpyHostToDevice, s));
+        w.sc.target = w.target;
+    }
+    GS_HIP(hipMemcpyAsync(w.ones, ones.data(), sizeof(double) * ntraj, hipMemcpyHostToDevice, s));
+    GS_HIP(hipMemcpyAsync(w.sc.s, ones.data(), sizeof(double) * ntraj, hipMemcpyHostToDevice, s));
     GS_HIP(hipStreamSynchronize(s));
     return BEC_OK;
 }
@@ -228,6 +250,39 @@ int final_report(GsWork &w, const bec_calc_params *params, int batch, hipStream_
     return BEC_OK;
 }
 
+// Pair mode: E, mu per component, N per component and the pair residual of
+// the final x-space state X.  With K_c, P_c the kinetic and potential sums and
+// N_c mu_c = K_c + P_c + (interaction of c), E = (K + P + sum_c N_c mu_c) / 2.
+int final_report2(GsWork &w, const bec_calc_params *params, int batch, hipStream_t s, bec_gs2_report *reps)
+{
+    const DevGroup &d = w.g.dev;
+    const int ntraj = d.ntraj;
+    GsScalars sc = w.sc;
+    sc.done = w.zero_done;
+    sc.s = w.ones;
+    GS_HIP(residual_pass(w, sc, 0.0, 0, s));
+    const int per = xy_wg_per_traj(d);
+    std::vector<double> h5(5 * (size_t)ntraj), mu(ntraj), res(ntraj);
+    GS_HIP(run_reduce_partials(w.ph, per, ntraj, 5, w.sums, s));
+    GS_HIP(hipMemcpyAsync(h5.data(), w.sums, sizeof(double) * 5 * ntraj, hipMemcpyDeviceToHost, s));
+    GS_HIP(hipMemcpyAsync(res.data(), sc.res, sizeof(double) * ntraj, hipMemcpyDeviceToHost, s));
+    GS_HIP(hipMemcpyAsync(mu.data(), sc.mu, sizeof(double) * ntraj, hipMemcpyDeviceToHost, s));
+    GS_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < batch; ++i) {
+        const bec_calc_params &c = params[i];
+        const double dV = c.dx * c.dy * c.dz;
+        const double *ha = &h5[5 * (2 * i)], *hb = &h5[5 * (2 * i + 1)];
+        bec_gs2_report &r = reps[i];
+        r.N_a = ha[1] * dV;
+        r.N_b = hb[1] * dV;
+        r.mu_a = mu[2 * i];
+        r.mu_b = mu[2 * i + 1];
+        r.E = (ha[2] + ha[3] + hb[2] + hb[3] + ha[0] + hb[0]) * dV / 2.0 / (r.N_a + r.N_b);
+        r.residual = std::max(res[2 * i], res[2 * i + 1]);
+    }
+    return BEC_OK;
+}
+
 } // namespace
 
 // Argument checks done before any device call.
@@ -251,16 +306,17 @@ int gs_validate(const bec_calc_params *params, int batch, const bec_gs_options *
 }
 
 int gs_solve(const bec_calc_params *params, int batch, const double *guesses, const bec_gs_options *opt,
-             hipStream_t s, double *out_x, bec_gs_report *reps)
+             hipStream_t s, double *out_x, bec_gs_report *reps, const double *frac_a, bec_gs2_report *reps2)
 {
     int vst = gs_validate(params, batch, opt);
     if (vst != BEC_OK) return vst;
     const bec_gs_options o = *opt;
     GsWork w;
-    int st = setup_work(w, params, batch, o.precision, s);
+    int st = setup_work(w, params, batch, o.precision, s, frac_a);
     if (st != BEC_OK) return st;
     DevGroup &d = w.g.dev;
-    const long n = w.g.cells * batch;
+    const int ntraj = d.ntraj;
+    const long n = w.g.cells * ntraj;
 
     // start: guesses or Thomas-Fermi (fillWithTFGroundState), normalised to N
     if (guesses) {
@@ -283,7 +339,7 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
     GS_HIP(hipEventCreate(&e0));
     GS_HIP(hipEventCreate(&e1));
     GS_HIP(hipEventRecord(e0, s));
-    std::vector<int> done(batch, 0);
+    std::vector<int> done(ntraj, 0);
     int it = 0;
     const int block = 16;
     while (it < o.max_iter) {
@@ -294,7 +350,7 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
             if (st != BEC_OK) break;
         }
         if (st != BEC_OK) break;
-        GS_HIP(hipMemcpyAsync(done.data(), w.sc.done, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+        GS_HIP(hipMemcpyAsync(done.data(), w.sc.done, sizeof(int) * ntraj, hipMemcpyDeviceToHost, s));
         GS_HIP(hipStreamSynchronize(s));
         if (std::all_of(done.begin(), done.end(), [](int v) { return v != 0; })) break;
     }
@@ -307,10 +363,30 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
     if (st != BEC_OK) return st;
     if (o.method == BEC_GS_SPLIT_STEP) GS_HIP(run_xy_fft(d, w.P, w.X, batch, -1, 1.0, s));
     GS_HIP(gs_apply_scale(d, w.X, w.sc, s));
-    std::vector<int> iters(batch);
-    GS_HIP(hipMemcpyAsync(iters.data(), w.sc.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
-    GS_HIP(hipMemcpyAsync(done.data(), w.sc.done, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    std::vector<int> iters(ntraj);
+    GS_HIP(hipMemcpyAsync(iters.data(), w.sc.iters, sizeof(int) * ntraj, hipMemcpyDeviceToHost, s));
+    GS_HIP(hipMemcpyAsync(done.data(), w.sc.done, sizeof(int) * ntraj, hipMemcpyDeviceToHost, s));
     GS_HIP(hipStreamSynchronize(s));
+    if (w.pair) {
+        std::vector<bec_gs2_report> r2(batch);
+        st = final_report2(w, params, batch, s, r2.data());
+        if (st != BEC_OK) return st;
+        if (out_x) {
+            st = download_complex(w.X, out_x, n, o.precision, s);
+            if (st != BEC_OK) return st;
+        }
+        bool all_conv = true, finite = true;
+        for (int i = 0; i < batch; ++i) {
+            r2[i].iterations = iters[2 * i];
+            r2[i].converged = done[2 * i] == 1 && done[2 * i + 1] == 1;
+            r2[i].elapsed_ms = ms;
+            if (!r2[i].converged) all_conv = false;
+            if (done[2 * i] == 2 || done[2 * i + 1] == 2 || !std::isfinite(r2[i].residual)) finite = false;
+            if (reps2) reps2[i] = r2[i];
+        }
+        if (!finite) return BEC_ERR_NONFINITE;
+        return all_conv ? BEC_OK : BEC_ERR_NOT_CONVERGED;
+    }
     std::vector<bec_gs_report> rr(batch);
     st = final_report(w, params, batch, s, rr.data());
     if (st != BEC_OK) return st;
@@ -385,7 +461,7 @@ extern "C" int bec_calculate_steady_state(bec_ctx *ctx, int group, const double 
     if (opt) o = *opt;
     else bec_default_gs_options(&o);
     bec_gs_report r{};
-    int st = bec::gs_solve(c, 1, guess_x, &o, s, out_x, &r);
+    int st = bec::gs_solve(c, 1, guess_x, &o, s, out_x, &r, nullptr, nullptr);
     if (rep) *rep = r;
     return st;
 }
@@ -404,7 +480,38 @@ extern "C" int bec_ground_states_batched(const bec_calc_params *params, int batc
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BEC_ERR_HIP;
         own = true;
     }
-    int st = bec::gs_solve(params, batch, guesses, &o, s, out_x, reps);
+    int st = bec::gs_solve(params, batch, guesses, &o, s, out_x, reps, nullptr, nullptr);
+    if (own) (void)hipStreamDestroy(s);
+    return st;
+}
+
+extern "C" int bec_ground_states_two_component(const bec_calc_params *params, int batch, const double *frac_a,
+                                               const double *guesses, const bec_gs_options *opt, void *stream,
+                                               double *out_x, bec_gs2_report *reps)
+{
+    bec_gs_options o;
+    if (opt) o = *opt;
+    else {
+        bec_default_gs_options(&o);
+        o.method = BEC_GS_GRADIENT;
+        o.rule = BEC_GS_RULE_RESIDUAL;
+    }
+    if (o.method != BEC_GS_GRADIENT || o.rule != BEC_GS_RULE_RESIDUAL) return BEC_ERR_INVALID;
+    int vst = bec::gs_validate(params, batch, &o);
+    if (vst != BEC_OK) return vst;
+    std::vector<double> fr(batch, 0.5);
+    if (frac_a)
+        for (int i = 0; i < batch; ++i) {
+            if (!(frac_a[i] > 0.0 && frac_a[i] < 1.0)) return BEC_ERR_INVALID;
+            fr[i] = frac_a[i];
+        }
+    hipStream_t s = (hipStream_t)stream;
+    bool own = false;
+    if (!s) {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BEC_ERR_HIP;
+        own = true;
+    }
+    int st = bec::gs_solve(params, batch, guesses, &o, s, out_x, nullptr, fr.data(), reps);
     if (own) (void)hipStreamDestroy(s);
     return st;
 }

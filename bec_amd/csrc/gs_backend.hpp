@@ -14,6 +14,8 @@ struct GsScalars {
     double *e_ref;    // last reference energy integral (not per particle)
     int *iters;       // iterations done
     int *done;        // converged flag
+    const double *target;  // per-trajectory atom number (nullptr: the record's N0)
+    int pair;         // 1: trajectories 2j, 2j+1 are the components of one coupled set
 };
 
 // Energy tables: e_xy[r][q] = (kx^2 + ky^2)/2, e_z[r][kz] = kz^2/2 (double)

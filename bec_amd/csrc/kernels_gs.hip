@@ -150,7 +150,10 @@ template <typename T, int N> struct OpGsForward : SkipDone {
     }
 };
 
-// h = K psi + (V + g11 |psi_s|^2) psi_s with psi_s = s psi.
+// h = K psi + (V + g |psi_s|^2) psi_s with psi_s = s psi and g = g11.
+// Pair mode (two-component ground state, report.tex:200-211): trajectory
+// 2 j + c holds component c of set j; g = g11 (c = 0) or g22 (c = 1) and the
+// partner's density adds g12 |psi_partner|^2.
 // partials {Re<psi_s,h>, <psi_s,psi_s>, Re<psi_s,K psi_s>, sum V n, sum n^2}
 template <typename T, int N> struct OpGsHam : SkipDone {
     static constexpr int NC = 1, SIGN_IN = -1, SIGN_OUT = 0, NSUM = 5;
@@ -159,20 +162,26 @@ template <typename T, int N> struct OpGsHam : SkipDone {
     cpx<T> *w;
     const double *scale;
     const TrajParams *P;
-    int rec_shift;
+    int rec_shift, pair;
+    long cells;
     __device__ const cpx<T> *src(int) const { return w; }
     __device__ cpx<T> *dst(int) const { return w; }
     __device__ __forceinline__ void apply(cpx<T> (&r)[1][N], int y, int z, int traj, long row, double *s) const
     {
         const TrajParams &p = P[traj >> rec_shift];
-        const T sc = (T)scale[traj], g11 = (T)p.g11;
+        const T sc = (T)scale[traj];
+        const T g = (T)((pair && (traj & 1)) ? p.g22 : p.g11), g12 = (T)p.g12;
+        const T scp = pair ? (T)scale[traj ^ 1] : (T)0;
+        const long poff = (traj & 1) ? -cells : cells;
         static_for<0, N>([&](auto X_) {
             constexpr int x = decltype(X_)::value;
             const cpx<T> ps = cscale<T>(psi[row + x], sc);
             const T n = cnorm<T>(ps);
             const T V = potential_t<T>(p, x, y, z);
+            T coup = (T)0;
+            if (pair) coup = g12 * cnorm<T>(cscale<T>(psi[row + poff + x], scp));
             const cpx<T> kp = r[0][x];
-            const cpx<T> h = cadd<T>(kp, cscale<T>(ps, V + g11 * n));
+            const cpx<T> h = cadd<T>(kp, cscale<T>(ps, V + g * n + coup));
             r[0][x] = h;
             s[0] += (double)ps.x * h.x + (double)ps.y * h.y;
             s[1] += (double)n;
@@ -283,14 +292,20 @@ __global__ void scalar_mu_kernel(const double *partial, int per, int ntraj, GsSc
     sc.mu[traj] = psum(partial, per, 5, traj, 0) / psum(partial, per, 5, traj, 1);
 }
 
+// Pair mode: both components of a set take max(r_a, r_b), so they stop together.
 __global__ void scalar_residual_kernel(const double *ph, const double *pr, int per, int ntraj, GsScalars sc,
-                                       double tol, int check)
+                                       double tol, int check, int pair)
 {
     const int traj = blockIdx.x * blockDim.x + threadIdx.x;
     if (traj >= ntraj || sc.done[traj]) return;
     const double nn = psum(ph, per, 5, traj, 1);
     const double rr = psum(pr, per, 1, traj, 0);
-    const double r = sqrt(rr / nn) / fabs(sc.mu[traj]);
+    double r = sqrt(rr / nn) / fabs(sc.mu[traj]);
+    if (pair) {
+        const int q = traj ^ 1;
+        const double rq = sqrt(psum(pr, per, 1, q, 0) / psum(ph, per, 5, q, 1)) / fabs(sc.mu[q]);
+        r = (r == r && rq == rq) ? fmax(r, rq) : rq + r;   // NaN propagates
+    }
     sc.res[traj] = r;
     if (!(r == r)) sc.done[traj] = 2;
     else if (check && r < tol) sc.done[traj] = 1;
@@ -303,7 +318,7 @@ __global__ void scalar_norm_kernel(const double *partial, int per, int ntraj, co
     if (traj >= ntraj || sc.done[traj]) return;
     const TrajParams &p = P[traj >> rec_shift];
     const double n = psum(partial, per, 1, traj, 0) * dv_of(p);
-    sc.s[traj] = sqrt(p.N0 / n);
+    sc.s[traj] = sqrt((sc.target ? sc.target[traj] : p.N0) / n);
     if (count_iter) sc.iters[traj] += 1;
     if (!(n == n)) sc.done[traj] = 2;
 }
@@ -453,6 +468,8 @@ hipError_t gs_xy_hamiltonian(const DevGroup &g, const void *psi, void *w, const 
         op.scale = sc.s;
         op.P = g.params;
         op.rec_shift = g.rec_shift;
+        op.pair = sc.pair;
+        op.cells = (long)g.nx * g.ny * g.nz;
         return launch_xy<T, N>(g, op, g.ntraj, partial, s);
     });
 }
@@ -484,7 +501,7 @@ hipError_t gs_scalar_residual(const DevGroup &g, const double *partial_h, const 
                               const GsScalars &sc, double tol, int check, hipStream_t s)
 {
     hipLaunchKernelGGL(scalar_residual_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial_h, partial_r,
-                       xy_wg_per_traj(g), g.ntraj, sc, tol, check);
+                       xy_wg_per_traj(g), g.ntraj, sc, tol, check, sc.pair);
     return hipGetLastError();
 }
 

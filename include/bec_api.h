@@ -147,6 +147,30 @@ int bec_ground_states_batched(const bec_calc_params *params, int batch, const do
                               const bec_gs_options *opt, void *hip_stream, double *out_x,
                               bec_gs_report *reps);
 
+/* Two-component ground states: the coupled imaginary-time GPEs that the
+ * reference describes but does not implement (docs/wigner_simulation.report/
+ * report.tex:200-211; its solver, wigner.cu:84-175, relaxes |1> only):
+ *   mu_a a = (K + V + g11|a|^2 + g12|b|^2) a,  mu_b b = (K + V + g22|b|^2 + g12|a|^2) b
+ * with dV sum|a|^2 = frac_a N and dV sum|b|^2 = (1 - frac_a) N, solved by the
+ * preconditioned gradient flow (method BEC_GS_GRADIENT, rule
+ * BEC_GS_RULE_RESIDUAL; opt NULL selects them).  frac_a may be NULL (1/2 each).
+ * guesses and out_x hold batch * 2 * cells complex values, x-space, laid out
+ * [set][component][cell]; guesses NULL gives the Thomas-Fermi start (the
+ * one-component TF shape of N atoms, scaled to each component's number). */
+typedef struct bec_gs2_report {
+    int iterations;
+    int converged;
+    double E;           /* total energy per particle */
+    double mu_a, mu_b;  /* per-component chemical potentials */
+    double N_a, N_b;
+    double residual;    /* max over components of ||H_c c - mu_c c|| / (|mu_c| ||c||) */
+    double elapsed_ms;  /* device time of the iterations (HIP events) */
+} bec_gs2_report;
+
+int bec_ground_states_two_component(const bec_calc_params *params, int batch, const double *frac_a,
+                                    const double *guesses, const bec_gs_options *opt, void *hip_stream,
+                                    double *out_x, bec_gs2_report *reps);
+
 /* Residual ||H psi - mu psi||/(|mu| ||psi||) and mu of an x-space state, FP64 on the GPU. */
 int bec_state_residual(bec_ctx *ctx, int group, const double *psi_x, double *residual, double *mu);
 

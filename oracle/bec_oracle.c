@@ -405,6 +405,159 @@ int orc_ground_state(const orc_calc *c, const orc_c *guess_x, int rule, double t
 }
 
 /* ------------------------------------------------------------------ */
+/* Two-component ground state: the coupled GPEs of
+ * docs/wigner_simulation.report/report.tex:200-211 in imaginary time. */
+
+static void kinetic_apply(const orc_calc *c, const double *k2, const orc_c *in, orc_c *out)
+{
+    memcpy(out, in, sizeof(orc_c) * c->cells);
+    orc_to_kspace(c, out, 1);
+    for (int i = 0; i < c->cells; i++) out[i] = cscale(out[i], k2[i]);
+    orc_to_xspace(c, out, 1);
+}
+
+void orc_apply_h2(const orc_calc *c, const orc_c *a, const orc_c *b, orc_c *ha, orc_c *hb)
+{
+    double *k2 = malloc(sizeof(double) * c->cells);
+    orc_ktable(c, k2);
+    kinetic_apply(c, k2, a, ha);
+    kinetic_apply(c, k2, b, hb);
+    for (int i = 0; i < c->cells; i++) {
+        double v = orc_potential(c, i), na = cnorm(a[i]), nb = cnorm(b[i]);
+        ha[i] = cadd(ha[i], cscale(a[i], v + c->g11 * na + c->g12 * nb));
+        hb[i] = cadd(hb[i], cscale(b[i], v + c->g22 * nb + c->g12 * na));
+    }
+    free(k2);
+}
+
+static double comp_residual(const orc_calc *c, const orc_c *x, const orc_c *h, double *mu_out)
+{
+    double num = 0, den = 0, r = 0;
+    for (int i = 0; i < c->cells; i++) {
+        num += x[i].re * h[i].re + x[i].im * h[i].im;
+        den += cnorm(x[i]);
+    }
+    double mu = num / den;
+    for (int i = 0; i < c->cells; i++) r += cnorm(csub(h[i], cscale(x[i], mu)));
+    if (mu_out) *mu_out = mu;
+    return sqrt(r / den) / fabs(mu);
+}
+
+double orc_residual2(const orc_calc *c, const orc_c *a, const orc_c *b, double *mu_a, double *mu_b)
+{
+    orc_c *ha = malloc(sizeof(orc_c) * c->cells), *hb = malloc(sizeof(orc_c) * c->cells);
+    orc_apply_h2(c, a, b, ha, hb);
+    double ra = comp_residual(c, a, ha, mu_a), rb = comp_residual(c, b, hb, mu_b);
+    free(ha); free(hb);
+    return ra > rb ? ra : rb;
+}
+
+double orc_energy2(const orc_calc *c, const orc_c *a, const orc_c *b)
+{
+    int n = c->cells;
+    orc_c *ka = malloc(sizeof(orc_c) * n), *kb = malloc(sizeof(orc_c) * n);
+    double *k2 = malloc(sizeof(double) * n);
+    orc_ktable(c, k2);
+    memcpy(ka, a, sizeof(orc_c) * n);
+    memcpy(kb, b, sizeof(orc_c) * n);
+    orc_to_kspace(c, ka, 1);
+    orc_to_kspace(c, kb, 1);
+    double kin = 0, rest = 0, norm = 0;
+    for (int i = 0; i < n; i++) {
+        double na = cnorm(a[i]), nb = cnorm(b[i]);
+        kin += k2[i] * (cnorm(ka[i]) + cnorm(kb[i]));
+        rest += orc_potential(c, i) * (na + nb) + c->g11 * na * na / 2.0 + c->g22 * nb * nb / 2.0
+                + c->g12 * na * nb;
+        norm += na + nb;
+    }
+    free(ka); free(kb); free(k2);
+    return (kin * (double)n + rest) / norm;   /* Parseval as in orc_energy_mu */
+}
+
+void orc_tf_state2(const orc_calc *c, double frac_a, orc_c *a, orc_c *b)
+{
+    orc_tf_state(c, a);
+    double s = particles(c, a);
+    double fa = sqrt(frac_a * c->N / s), fb = sqrt((1.0 - frac_a) * c->N / s);
+    for (int i = 0; i < c->cells; i++) {
+        b[i] = cscale(a[i], fb);
+        a[i] = cscale(a[i], fa);
+    }
+}
+
+static void renorm(const orc_calc *c, orc_c *x, double target)
+{
+    double s = sqrt(target / particles(c, x));
+    for (int i = 0; i < c->cells; i++) x[i] = cscale(x[i], s);
+}
+
+int orc_ground_state2(const orc_calc *c, double frac_a, const orc_c *guess_a, const orc_c *guess_b,
+                      double tol, int max_iter, double alpha, double tau, double beta,
+                      orc_c *out_a, orc_c *out_b, orc_gs2_report *rep)
+{
+    int n = c->cells;
+    size_t bytes = sizeof(orc_c) * n;
+    orc_c *x[2], *pv[2], *h[2];
+    double *pre = malloc(sizeof(double) * n);
+    orc_ktable(c, pre);
+    for (int i = 0; i < n; i++) pre[i] = 1.0 / (alpha + pre[i]);
+    for (int k = 0; k < 2; k++) {
+        x[k] = malloc(bytes); pv[k] = malloc(bytes); h[k] = malloc(bytes);
+    }
+    const double target[2] = {frac_a * c->N, (1.0 - frac_a) * c->N};
+    if (guess_a && guess_b) {
+        memcpy(x[0], guess_a, bytes);
+        memcpy(x[1], guess_b, bytes);
+    } else {
+        orc_tf_state2(c, frac_a, x[0], x[1]);
+    }
+    for (int k = 0; k < 2; k++) {
+        renorm(c, x[k], target[k]);
+        memcpy(pv[k], x[k], bytes);
+    }
+    int it = 0, conv = 0;
+    double res = 0, mu[2] = {0, 0};
+    for (;;) {
+        orc_apply_h2(c, x[0], x[1], h[0], h[1]);
+        double r0 = comp_residual(c, x[0], h[0], &mu[0]), r1 = comp_residual(c, x[1], h[1], &mu[1]);
+        res = r0 > r1 ? r0 : r1;
+        if (!(res == res)) break;
+        if (res < tol) { conv = 1; break; }
+        if (it >= max_iter) break;
+        for (int k = 0; k < 2; k++) {
+            /* d = P (H - mu) x, applied in k-space */
+            for (int i = 0; i < n; i++) h[k][i] = csub(h[k][i], cscale(x[k][i], mu[k]));
+            orc_to_kspace(c, h[k], 1);
+            for (int i = 0; i < n; i++) h[k][i] = cscale(h[k][i], pre[i]);
+            orc_to_xspace(c, h[k], 1);
+            for (int i = 0; i < n; i++) {
+                orc_c cur = x[k][i];
+                orc_c nw = cadd(csub(cur, cscale(h[k][i], tau)), cscale(csub(cur, pv[k][i]), beta));
+                pv[k][i] = cur;
+                x[k][i] = nw;
+            }
+            renorm(c, x[k], target[k]);
+        }
+        it++;
+    }
+    if (out_a) memcpy(out_a, x[0], bytes);
+    if (out_b) memcpy(out_b, x[1], bytes);
+    if (rep) {
+        rep->iterations = it;
+        rep->converged = conv;
+        rep->E = orc_energy2(c, x[0], x[1]);
+        rep->mu_a = mu[0];
+        rep->mu_b = mu[1];
+        rep->N_a = particles(c, x[0]);
+        rep->N_b = particles(c, x[1]);
+        rep->residual = res;
+    }
+    for (int k = 0; k < 2; k++) { free(x[k]); free(pv[k]); free(h[k]); }
+    free(pre);
+    return conv ? 0 : 1;
+}
+
+/* ------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), restated from the
  * published round function.                                           */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])

@@ -90,6 +90,34 @@ void orc_energy_mu(const orc_calc *c, const orc_c *psi_x, double *E, double *mu,
 double orc_residual(const orc_calc *c, const orc_c *psi_x, double *mu_out);
 void orc_gs_iterate(const orc_calc *c, orc_c *a_k, int n_iter, double *E_ref_out);
 
+/* ---- two-component ground state (docs/wigner_simulation.report/report.tex:200-211) ----
+ * mu_a a = (K + V + g11|a|^2 + g12|b|^2) a, mu_b b = (K + V + g22|b|^2 + g12|a|^2) b
+ * with dV sum |a|^2 = frac_a N and dV sum |b|^2 = (1 - frac_a) N.  The constant
+ * hyperfine offset of |2> shifts mu_b only and is left out. */
+typedef struct {
+    int iterations;
+    int converged;
+    double E;          /* total energy per particle */
+    double mu_a, mu_b; /* per-component Rayleigh quotients */
+    double N_a, N_b;
+    double residual;   /* max over components of ||H_c c - mu_c c|| / (|mu_c| ||c||) */
+} orc_gs2_report;
+
+/* H_a a and H_b b (x-space, one pair) */
+void orc_apply_h2(const orc_calc *c, const orc_c *a, const orc_c *b, orc_c *ha, orc_c *hb);
+double orc_residual2(const orc_calc *c, const orc_c *a, const orc_c *b, double *mu_a, double *mu_b);
+double orc_energy2(const orc_calc *c, const orc_c *a, const orc_c *b);
+/* Thomas-Fermi start: the one-component TF shape of N atoms (g11), scaled to
+ * frac_a N and (1 - frac_a) N */
+void orc_tf_state2(const orc_calc *c, double frac_a, orc_c *a, orc_c *b);
+/* Preconditioned imaginary-time gradient flow with momentum:
+ *   c <- c - tau P (H_c - mu_c) c + beta (c - c_prev), then each component
+ *   renormalised, P = (alpha + k^2/2)^-1; stops at residual < tol (checked
+ *   every iteration).  guesses may be NULL (TF start). */
+int orc_ground_state2(const orc_calc *c, double frac_a, const orc_c *guess_a, const orc_c *guess_b,
+                      double tol, int max_iter, double alpha, double tau, double beta,
+                      orc_c *out_a, orc_c *out_b, orc_gs2_report *rep);
+
 /* ---- noise + ensembles ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_noise_cell(uint64_t seed, uint32_t traj, uint32_t cell, uint32_t stream, double out[4]);

@@ -46,6 +46,12 @@ class GSReport(C.Structure):
                 ("N", C.c_double), ("residual", C.c_double)]
 
 
+class GS2Report(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("E", C.c_double),
+                ("mu_a", C.c_double), ("mu_b", C.c_double), ("N_a", C.c_double), ("N_b", C.c_double),
+                ("residual", C.c_double)]
+
+
 class Params2D(C.Structure):
     """File-scope parameters of programs/bec_wigner_2d.cpp/wigner.cpp:106-136."""
     _fields_ = [("nvx", C.c_int), ("nvy", C.c_int), ("xmax", C.c_double), ("ymax", C.c_double),
@@ -90,6 +96,13 @@ def lib():
             "orc_energy_mu": (None, [C.POINTER(Calc), _P, _D, _D, _D]),
             "orc_residual": (C.c_double, [C.POINTER(Calc), _P, _D]),
             "orc_gs_iterate": (None, [C.POINTER(Calc), _P, C.c_int, _D]),
+            "orc_apply_h2": (None, [C.POINTER(Calc), _P, _P, _P, _P]),
+            "orc_residual2": (C.c_double, [C.POINTER(Calc), _P, _P, _D, _D]),
+            "orc_energy2": (C.c_double, [C.POINTER(Calc), _P, _P]),
+            "orc_tf_state2": (None, [C.POINTER(Calc), C.c_double, _P, _P]),
+            "orc_ground_state2": (C.c_int, [C.POINTER(Calc), C.c_double, _P, _P, C.c_double, C.c_int,
+                                            C.c_double, C.c_double, C.c_double, _P, _P,
+                                            C.POINTER(GS2Report)]),
             "orc_philox4x32_10": (None, [_P, _P, _P]),
             "orc_noise_cell": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
             "orc_init_ensembles": (None, [C.POINTER(Calc), _P, C.c_uint64, _P, _P]),
@@ -226,6 +239,44 @@ def residual(c, psi_x):
     mu = C.c_double()
     r = lib().orc_residual(C.byref(c), _ptr(_c128(psi_x)), C.byref(mu))
     return r, mu.value
+
+
+def apply_h2(c, a, b):
+    ha = np.empty(shape(c), dtype=np.complex128)
+    hb = np.empty(shape(c), dtype=np.complex128)
+    lib().orc_apply_h2(C.byref(c), _ptr(_c128(a)), _ptr(_c128(b)), _ptr(ha), _ptr(hb))
+    return ha, hb
+
+
+def residual2(c, a, b):
+    ma, mb = C.c_double(), C.c_double()
+    r = lib().orc_residual2(C.byref(c), _ptr(_c128(a)), _ptr(_c128(b)), C.byref(ma), C.byref(mb))
+    return r, ma.value, mb.value
+
+
+def energy2(c, a, b):
+    return lib().orc_energy2(C.byref(c), _ptr(_c128(a)), _ptr(_c128(b)))
+
+
+def tf_state2(c, frac_a=0.5):
+    a = np.empty(shape(c), dtype=np.complex128)
+    b = np.empty(shape(c), dtype=np.complex128)
+    lib().orc_tf_state2(C.byref(c), frac_a, _ptr(a), _ptr(b))
+    return a, b
+
+
+def ground_state2(c, frac_a=0.5, guess=None, tol=1e-8, max_iter=5000, alpha=None, tau=0.5, beta=0.6):
+    """Two-component ground state (gradient flow); returns (a, b, report)."""
+    a = np.empty(shape(c), dtype=np.complex128)
+    b = np.empty(shape(c), dtype=np.complex128)
+    rep = GS2Report()
+    ga = gb = None
+    if guess is not None:
+        ga, gb = _c128(guess[0]), _c128(guess[1])
+    lib().orc_ground_state2(C.byref(c), frac_a, None if ga is None else _ptr(ga), None if gb is None else _ptr(gb),
+                            tol, max_iter, c.mu if alpha is None else alpha, tau, beta, _ptr(a), _ptr(b),
+                            C.byref(rep))
+    return a, b, rep
 
 
 def philox(ctr, key):
