@@ -173,7 +173,7 @@ hipError_t residual_pass(GsWork &w, const GsScalars &sc, double tol, int check, 
     GsTables t{w.exy, w.ez};
     hipError_t e;
     if ((e = gs_xy_forward(d, w.X, w.W, sc, s)) != hipSuccess) return e;
-    if ((e = gs_z_diag(d, w.W, sc, t, 0, 0.0, s)) != hipSuccess) return e;
+    if ((e = gs_z_diag(d, w.W, sc, t, 0, 0.0, nullptr, s)) != hipSuccess) return e;
     if ((e = gs_xy_hamiltonian(d, w.X, w.W, sc, w.ph, s)) != hipSuccess) return e;
     if ((e = gs_scalar_mu(d, w.ph, sc, s)) != hipSuccess) return e;
     if ((e = gs_xy_residual(d, w.X, w.W, sc, w.pr, s)) != hipSuccess) return e;
@@ -204,7 +204,8 @@ int gradient_iteration(GsWork &w, const bec_gs_options &o, int it, double alpha,
     GsTables t{w.exy, w.ez};
     const int check = o.rule == BEC_GS_RULE_RESIDUAL && (it % std::max(1, o.check_every) == 0);
     GS_HIP(residual_pass(w, w.sc, o.tol, check, s));
-    GS_HIP(gs_z_diag(d, w.W, w.sc, t, 1, alpha, s));
+    // preconditioner shift: the option, else each record's Thomas-Fermi mu
+    GS_HIP(gs_z_diag(d, w.W, w.sc, t, 1, alpha, alpha > 0 ? nullptr : w.mu_tf, s));
     GS_HIP(gs_xy_update(d, w.X, w.P, w.W, w.sc, o.tau, o.beta, w.pn, s));
     GS_HIP(gs_scalar_norm(d, w.pn, w.sc, s));
     return BEC_OK;
@@ -326,8 +327,7 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
         GS_HIP(gs_tf_start(d, w.X, w.mu_tf, s));
     }
     GS_HIP(gs_norm_only(d, w.X, w.sc, w.pn, s));
-    double alpha = o.alpha;
-    if (!(alpha > 0)) alpha = params[0].mu;
+    const double alpha = o.alpha > 0 ? o.alpha : 0.0;   // 0: per-record mu
     if (o.method == BEC_GS_SPLIT_STEP) {
         GS_HIP(gs_xy_forward(d, w.X, w.P, w.sc, s));     // x -> hybrid, renormalised
         GS_HIP(gs_apply_scale(d, w.X, w.sc, s));          // keeps X consistent, s <- 1

@@ -39,9 +39,10 @@ hipError_t gs_scalar_ref(const DevGroup &g, const double *partial, const GsScala
 
 // ---- gradient flow / residual machinery (x-space iterate psi)
 hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s);
-// variant 0: multiply by k^2/2 / nz; 1: by 1 / ((alpha + k^2/2) nz)
+// variant 0: multiply by k^2/2 / nz; 1: by 1 / ((alpha + k^2/2) nz), alpha
+// taken per record from alpha_rec when it is not nullptr
 hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTables &t, int variant,
-                     double alpha, hipStream_t s);
+                     double alpha, const double *alpha_rec, hipStream_t s);
 // h = K psi + (V + g11 |psi|^2) psi; partials {Re<psi,h>, <psi,psi>, <psi,K psi>, sum V n, sum n^2}
 hipError_t gs_xy_hamiltonian(const DevGroup &g, const void *psi, void *w, const GsScalars &sc,
                              double *partial, hipStream_t s);

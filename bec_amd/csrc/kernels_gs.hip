@@ -51,6 +51,7 @@ template <typename T, int NZ, int R1> struct OpGsZDiag : SkipDone {
     static constexpr int NW = NZ / R1, KPT = R1 / NW;
     cpx<T> *w;
     const double *exy, *ez;
+    const double *alpha_rec;   // per-record preconditioner shift (nullptr: alpha)
     int nxy, rec_shift, variant;
     double alpha;
     __device__ const cpx<T> *src(int) const { return w; }
@@ -60,12 +61,13 @@ template <typename T, int NZ, int R1> struct OpGsZDiag : SkipDone {
         const int rec = traj >> rec_shift;
         const double e0 = exy[(long)rec * nxy + q];
         const double *ezr = ez + (long)rec * NZ;
+        const double al = alpha_rec ? alpha_rec[rec] : alpha;
 #pragma unroll
         for (int i = 0; i < KPT; ++i)
 #pragma unroll
             for (int m = 0; m < NW; ++m) {
                 const double e = e0 + ezr[kk0 + i + R1 * m];
-                const T f = (T)(variant == 0 ? e / NZ : 1.0 / ((alpha + e) * NZ));
+                const T f = (T)(variant == 0 ? e / NZ : 1.0 / ((al + e) * NZ));
                 X[i][m] = cscale<T>(X[i][m], f);
             }
     }
@@ -380,7 +382,7 @@ hipError_t gs_z_imag(const DevGroup &g, void *psi, const GsScalars &sc, void *kc
 }
 
 hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTables &t, int variant, double alpha,
-                     hipStream_t s)
+                     const double *alpha_rec, hipStream_t s)
 {
     int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
     return dispatch(g, [&](auto Tt, auto, auto NZt) -> hipError_t {
@@ -396,6 +398,7 @@ hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTa
         op.rec_shift = g.rec_shift;
         op.variant = variant;
         op.alpha = alpha;
+        op.alpha_rec = alpha_rec;
         return with_pw(pw, [&](auto PWt) { return launch_z<T, NZ, decltype(PWt)::value, +1, true>(g, op, g.ntraj, s); });
     });
 }

@@ -1,0 +1,244 @@
+"""Learned ground-state start: parameter sampling, network inputs, training and
+the time-to-residual comparison against the Thomas-Fermi start.
+
+    python -m bec_amd.surrogate [--train 2000 --test 200 --epochs 60 --seed 7 --out FILE]
+
+One run generates the two-component ground states with the library's own
+solver (bec_ground_states_two_component, FP64, gradient flow), trains the
+network of include/bec_fno.h on them with its own BF16 MFMA kernels, and then
+solves every held-out parameter set twice to residual 1e-6: from the Thomas-
+Fermi start and from the network's guess.  Prints progress lines and one JSON
+summary line.
+
+Parameter sets (uniform, seeded): N in [5e3, 1e5], axial trap frequency in
+[5, 20] Hz, radial in [50, 150] Hz, a12 in [95, 100] Bohr radii; a11 = 100.4,
+a22 = 95.0; half of the atoms in each component; lattice 128 x 16 x 16.
+
+Network inputs per cell (16 channels): the one-component Thomas-Fermi
+amplitude sqrt(max(mu - V, 0) / mu), V / mu, the cell coordinates scaled to
+[-1, 1] (x, y, z), the four parameters scaled to [-1, 1], and zeros.  Outputs:
+the two component amplitudes in units of sqrt(mu / g11).  The lattice box
+scales with mu (xmax = 1.2 sqrt(2 mu)), so these targets are O(1) over the
+whole parameter range.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import bec_amd as B  # noqa: E402
+from bec_amd import fno as F  # noqa: E402
+
+RANGES = (("N", 5e3, 1e5), ("fz", 5.0, 20.0), ("fr", 50.0, 150.0), ("a12", 95.0, 100.0))
+C_IN, C_OUT = 16, 2
+NET = dict(nz=128, ny=16, nx=16, mz=16, my=16, mx=16, width=32, layers=4, c_in=C_IN, c_out=C_OUT, hidden=64)
+
+
+def sample_params(n, seed):
+    rng = np.random.default_rng(seed)
+    lo = np.array([r[1] for r in RANGES])
+    hi = np.array([r[2] for r in RANGES])
+    return lo + (hi - lo) * rng.random((n, len(RANGES)))
+
+
+def scaled(theta):
+    lo = np.array([r[1] for r in RANGES])
+    hi = np.array([r[2] for r in RANGES])
+    return 2.0 * (np.asarray(theta) - lo) / (hi - lo) - 1.0
+
+
+def calc_for(theta, nz=128, nxy=16):
+    N, fz, fr, a12 = theta
+    m = B.default_model(N=int(round(N)), fx=fr, fy=fr, fz=fz, a12=a12, nvx=nxy, nvy=nxy, nvz=nz, ne=1)
+    return B.derive(m)
+
+
+def features(c, theta):
+    """[cells][16] float32 inputs for one parameter set (cell = z ny nx + y nx + x)."""
+    x = -c.xmax + c.dx * np.arange(c.nvx)
+    y = -c.ymax + c.dy * np.arange(c.nvy)
+    z = -c.zmax + c.dz * np.arange(c.nvz)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    V = 0.5 * (X ** 2 + Y ** 2 + Z ** 2 / c.lambda_ ** 2)
+    f = np.zeros((c.nvz, c.nvy, c.nvx, C_IN), dtype=np.float32)
+    f[..., 0] = np.sqrt(np.maximum(c.mu - V, 0.0) / c.mu)
+    f[..., 1] = V / c.mu
+    f[..., 2] = X / c.xmax
+    f[..., 3] = Y / c.ymax
+    f[..., 4] = Z / c.zmax
+    f[..., 5:9] = scaled(theta)
+    return f.reshape(-1, C_IN)
+
+
+def amp_unit(c):
+    return math.sqrt(c.mu / c.g11)
+
+
+def targets(c, state):
+    """state [2][nz][ny][nx] complex (x-space ground state) -> [cells][2] float32."""
+    u = amp_unit(c)
+    return np.stack([state[0].real.ravel() / u, state[1].real.ravel() / u], axis=-1).astype(np.float32)
+
+
+def guesses(c, pred):
+    """Network output [cells][2] -> solver guess [2][cells] complex."""
+    u = amp_unit(c)
+    return (np.asarray(pred, dtype=np.float64).T * u).astype(np.complex128)
+
+
+def solve(calcs, guess=None, tol=1e-6, max_iter=20000):
+    out, reps, st = B.ground_states_two_component(calcs, guesses=guess, tol=tol, max_iter=max_iter,
+                                                  precision=B.FP64)
+    return out, reps, st
+
+
+def generate(thetas, tol=1e-7, chunk=200, log=print):
+    """Ground states of many parameter sets, solved in batches on the GPU."""
+    n = len(thetas)
+    X = np.empty((n, 128 * 256, C_IN), dtype=np.float32)
+    Y = np.empty((n, 128 * 256, C_OUT), dtype=np.float32)
+    for s in range(0, n, chunk):
+        part = thetas[s:s + chunk]
+        calcs = [calc_for(t) for t in part]
+        out, reps, st = solve(calcs, tol=tol, max_iter=40000)
+        if st != 0:
+            raise RuntimeError(f"ground states did not converge (status {st})")
+        for i, (c, t) in enumerate(zip(calcs, part)):
+            X[s + i] = features(c, t)
+            Y[s + i] = targets(c, out[i])
+        log(f"generated {min(s + chunk, n)}/{n} ground states "
+            f"(iterations max {max(r['iterations'] for r in reps)})")
+    return X, Y
+
+
+def train(net, X, Y, epochs, batch=16, lr=1e-3, lr_min=1e-5, seed=0, log=print, time_limit=None):
+    """Adam with a cosine learning rate; the whole set is resident on the device."""
+    n = X.shape[0]
+    dX = F.DeviceArray(X.shape, np.float32, X)
+    dY = F.DeviceArray(Y.shape, np.float32, Y)
+    per_x = X.shape[1] * X.shape[2] * 4
+    per_y = Y.shape[1] * Y.shape[2] * 4
+    rng = np.random.default_rng(seed)
+    nb = n // batch
+    total = epochs * nb
+    step = 0
+    hist = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(nb)
+        acc = 0.0
+        for j in order:
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(1, total)))
+            vx = F.DeviceView(dX, j * batch * per_x)
+            vy = F.DeviceView(dY, j * batch * per_y)
+            acc += net.loss_backward(vx, vy, batch)
+            net.adam_step(cur)
+            step += 1
+        hist.append(acc / nb)
+        el = time.perf_counter() - t0
+        log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.3e} lr {cur:.2e} {el:.0f}s")
+        if time_limit and el > time_limit:
+            log("time limit reached")
+            break
+    dX.free()
+    dY.free()
+    return hist, time.perf_counter() - t0
+
+
+def predict(net, X, batch=16):
+    out = np.empty((X.shape[0], X.shape[1], C_OUT), dtype=np.float32)
+    for s in range(0, X.shape[0], batch):
+        out[s:s + batch] = net.predict(X[s:s + batch])
+    return out
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def evaluate(net, thetas, X, Y, tol=1e-6, log=print):
+    """Per held-out set: solve from TF and from the network guess (one set per solve)."""
+    rows = []
+    for i, t in enumerate(thetas):
+        c = calc_for(t)
+        _, r_tf, st_tf = solve([c], tol=tol)
+        t0 = time.perf_counter()
+        pred = net.predict(X[i:i + 1])[0]
+        t_inf = (time.perf_counter() - t0) * 1e3
+        g = guesses(c, pred)[None]
+        _, r_nn, st_nn = solve([c], guess=g, tol=tol)
+        rows.append({"theta": [float(v) for v in t], "tf_iters": r_tf[0]["iterations"],
+                     "tf_ms": r_tf[0]["elapsed_ms"], "tf_residual": r_tf[0]["residual"],
+                     "nn_iters": r_nn[0]["iterations"], "nn_ms": r_nn[0]["elapsed_ms"],
+                     "nn_residual": r_nn[0]["residual"], "inference_ms": t_inf,
+                     "guess_rel_l2": rel_l2(pred, Y[i]), "converged": bool(st_tf == 0 and st_nn == 0)})
+        if (i + 1) % 50 == 0:
+            log(f"evaluated {i + 1}/{len(thetas)}")
+    return rows
+
+
+def summary(rows):
+    tf_ms = np.array([r["tf_ms"] for r in rows])
+    nn_ms = np.array([r["nn_ms"] + r["inference_ms"] for r in rows])
+    return {
+        "held_out": len(rows),
+        "converged_all": all(r["converged"] for r in rows),
+        "tf_iterations_mean": float(np.mean([r["tf_iters"] for r in rows])),
+        "nn_iterations_mean": float(np.mean([r["nn_iters"] for r in rows])),
+        "tf_ms_mean": float(tf_ms.mean()),
+        "nn_ms_mean_incl_inference": float(nn_ms.mean()),
+        "inference_ms_mean": float(np.mean([r["inference_ms"] for r in rows])),
+        "speedup_mean_time": float(tf_ms.mean() / nn_ms.mean()),
+        "speedup_median_per_set": float(np.median(tf_ms / nn_ms)),
+        "speedup_iterations": float(np.sum([r["tf_iters"] for r in rows]) / np.sum([r["nn_iters"] for r in rows])),
+        "final_residual_max": {"tf": float(max(r["tf_residual"] for r in rows)),
+                               "nn": float(max(r["nn_residual"] for r in rows))},
+        "guess_rel_l2_mean": float(np.mean([r["guess_rel_l2"] for r in rows])),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=2000)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--time-limit", type=float, default=None, help="training wall-clock limit, seconds")
+    ap.add_argument("--out", default=None, help="write the JSON summary (with per-set rows) here")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    thetas = sample_params(a.train + a.test, a.seed)
+    t0 = time.perf_counter()
+    X, Y = generate(thetas, log=log)
+    t_gen = time.perf_counter() - t0
+    net = F.FNO(seed=a.seed, max_batch=a.batch, **NET)
+    log(f"network parameters {net.n_params}")
+    hist, t_train = train(net, X[:a.train], Y[:a.train], a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log,
+                          time_limit=a.time_limit)
+    rows = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log)
+    res = {"config": dict(NET, train=a.train, test=a.test, epochs_run=len(hist), batch=a.batch, lr=a.lr,
+                          seed=a.seed),
+           "generate_s": t_gen, "train_s": t_train, "loss_first": hist[0], "loss_last": hist[-1],
+           "train_loss": hist, "summary": summary(rows)}
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(dict(res, rows=rows), f)
+    print(json.dumps({k: v for k, v in res.items() if k != "train_loss"}))
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -9,11 +9,17 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    with open(os.path.join(ROOT, "include", "bec_api.h")) as f:
-        text = f.read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(bec_[a-z0-9_]+)\s*\(", text)))
+HEADERS = ("bec_api.h", "bec_fno.h")
+
+
+def _declared(headers=HEADERS):
+    names = set()
+    for h in headers:
+        with open(os.path.join(ROOT, "include", h)) as f:
+            text = f.read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(bec_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,8 +32,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_binding_covers_the_header():
-    from bec_amd import api
-    missing = [n for n in _declared() if n not in api.EXPORTS]
+    from bec_amd import api, fno
+    missing = [n for n in _declared(("bec_api.h",)) if n not in api.EXPORTS]
+    missing += [n for n in _declared(("bec_fno.h",)) if n not in fno.EXPORTS]
     assert not missing, missing
 
 
@@ -86,6 +93,15 @@ def test_invalid_arguments_rejected_on_cpu():
     o = bec_amd.gs_options()
     bad = bec_amd.derive(bec_amd.default_model(nvz=32))
     assert L.bec_ground_states_batched(C.byref(bad), 1, None, C.byref(o), None, None, reps) == api.ERR_SHAPE
+    # two-component ground state: only the gradient flow with the residual rule; fractions in (0, 1)
+    reps2 = (api.Gs2Report * 1)()
+    o = bec_amd.gs_options(method=bec_amd.GS_SPLIT_STEP, rule=bec_amd.GS_RULE_RESIDUAL)
+    assert L.bec_ground_states_two_component(C.byref(cc), 1, None, None, C.byref(o), None, None, reps2) == \
+        api.ERR_INVALID
+    o = bec_amd.gs_options(method=bec_amd.GS_GRADIENT, rule=bec_amd.GS_RULE_RESIDUAL)
+    bad_frac = (C.c_double * 1)(1.0)
+    assert L.bec_ground_states_two_component(C.byref(cc), 1, bad_frac, None, C.byref(o), None, None, reps2) == \
+        api.ERR_INVALID
     # context-less calls
     assert L.bec_propagate(None, 1, 0.0, 0, None) == api.ERR_INVALID
     assert L.bec_observables(None, 0, None) == api.ERR_INVALID
@@ -96,3 +112,25 @@ def test_default_gs_options_are_the_reference_method():
     import bec_amd
     o = bec_amd.gs_options()
     assert (o.method, o.rule, o.tol, o.check_every) == (bec_amd.GS_SPLIT_STEP, bec_amd.GS_RULE_REFERENCE, 1e-6, 1)
+
+
+def test_surrogate_entry_points_validate_on_cpu():
+    from bec_amd import api, fno
+    L = fno.flib()
+    d = fno.GemmDesc()
+    assert L.bec_gemm(None, None) == api.ERR_INVALID
+    assert L.bec_gemm(C.byref(d), None) == api.ERR_INVALID          # NULL operands
+    d.A = d.B = d.C = C.c_void_p(16)
+    d.M = d.N = d.K = d.batch = 1
+    d.ksplit = 4                                                     # split K without a workspace
+    assert L.bec_gemm(C.byref(d), None) == api.ERR_INVALID
+    h = C.c_void_p()
+    good = dict(nz=128, ny=16, nx=16, mz=16, my=16, mx=16, width=32, layers=4, c_in=16, c_out=2, hidden=64,
+                max_batch=16)
+    for bad in (dict(mz=3), dict(mz=0), dict(mx=32), dict(width=0), dict(max_batch=0)):
+        cfg = fno.FnoConfig(**dict(good, **bad))
+        assert L.bec_fno_create(C.byref(cfg), 0, C.byref(h)) == api.ERR_INVALID, bad
+        assert not h.value
+    cfg = fno.FnoConfig(**dict(good, width=24))                      # channel maps need powers of two
+    assert L.bec_fno_create(C.byref(cfg), 0, C.byref(h)) == api.ERR_SHAPE
+    assert L.bec_fno_forward(None, 1, None, None, None) == api.ERR_INVALID
