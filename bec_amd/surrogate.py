@@ -165,15 +165,22 @@ def rel_l2(a, b):
 
 
 def evaluate(net, thetas, X, Y, tol=1e-6, log=print):
-    """Per held-out set: solve from TF and from the network guess (one set per solve)."""
+    """Per held-out set: solve from TF and from the network guess (one set per solve).
+    inference_ms is the host wall clock of the whole guess: input features,
+    upload, forward pass, download and conversion to the solver's layout."""
     rows = []
+    cells = X.shape[1]
+    dx = F.DeviceArray((1, cells, C_IN), np.float32)
+    dy = F.DeviceArray((1, cells, C_OUT), np.float32)
     for i, t in enumerate(thetas):
         c = calc_for(t)
         _, r_tf, st_tf = solve([c], tol=tol)
         t0 = time.perf_counter()
-        pred = net.predict(X[i:i + 1])[0]
-        t_inf = (time.perf_counter() - t0) * 1e3
+        dx.upload(features(c, t)[None])
+        net.forward(dx, dy, 1)
+        pred = dy.download()[0]
         g = guesses(c, pred)[None]
+        t_inf = (time.perf_counter() - t0) * 1e3
         _, r_nn, st_nn = solve([c], guess=g, tol=tol)
         rows.append({"theta": [float(v) for v in t], "tf_iters": r_tf[0]["iterations"],
                      "tf_ms": r_tf[0]["elapsed_ms"], "tf_residual": r_tf[0]["residual"],

@@ -54,6 +54,50 @@ def load_traffic():
     return {k: v["bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
 
 
+def uneven_groups(B, psi_a, steps, autotune=True):
+    """256 trajectories at 128x16x16 (N = 30000) and 128 at 256x32x32 (N = 70000) stepped together
+    in one context (captured step blocks with a concurrent branch per group, autotuned schedules)
+    against the plain per-grid launch (one context per grid, one after the other, default
+    schedule, no graphs).  Same physics as the headline run; no sampling."""
+    ca = B.derive(B.default_model(**config1(ne=256)))
+    kb = config1(ne=128, N=70000, nvx=32, nvy=32, nvz=256)
+    cb1 = B.derive(B.default_model(**dict(kb, ne=1)))
+    with B.Solver(cb1, precision=B.FP64) as g:
+        psi_b, rep_b, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
+                                                   precision=B.FP64, max_iter=20000)
+    cb = B.derive(B.default_model(**kb))
+    ntot = ca.ne + cb.ne
+
+    def timed(solver, n):
+        solver.propagate(50)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.propagate(n)
+        solver.synchronize()
+        return time.perf_counter() - t0
+
+    s = B.Solver(ca, precision=B.FP32, groups=[ca, cb])
+    s.init_evolution(psi_a, SEED, group=0, equilibrate=False)
+    s.init_evolution(psi_b, SEED, group=1, equilibrate=False)
+    scheds = [s.autotune(steps=8, group=g)[0] if autotune else s.schedule(g) for g in (0, 1)]
+    el = timed(s, steps)
+    s.close()
+    plain = {}
+    for name, c, psi in (("128x16x16", ca, psi_a), ("256x32x32", cb, psi_b)):
+        p = B.Solver(c, precision=B.FP32)
+        p.set_graphs(False)
+        p.init_evolution(psi, SEED, equilibrate=False)
+        plain[name] = {"trajectories": c.ne, "seconds": timed(p, steps)}
+        plain[name]["traj_steps_per_s"] = c.ne * steps / plain[name]["seconds"]
+        p.close()
+    t_plain = sum(v["seconds"] for v in plain.values())
+    return {"workload": "256 traj x 128x16x16 (N=30000) + 128 traj x 256x32x32 (N=70000), complex64, one context",
+            "steps": steps, "one_launch_traj_steps_per_s": ntot * steps / el,
+            "plain_per_grid_traj_steps_per_s": ntot * steps / t_plain, "speedup_over_plain": t_plain / el,
+            "plain_per_grid": plain, "schedules": {"128x16x16": scheds[0], "256x32x32": scheds[1]},
+            "ground_state_256x32x32": {"iterations": rep_b["iterations"], "residual": rep_b["residual"]}}
+
+
 def cpu_baseline(psi, seconds):
     """FP64 oracle split-step (same algorithm and seeds) timed on this host's cores."""
     from oracle import oracle as O
@@ -86,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-groups comparison")
+    ap.add_argument("--uneven-steps", type=int, default=400)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories are independent, "
@@ -156,6 +202,8 @@ def main():
             "passes_per_step": 2, "schedule": sched_big,
             "copy_kernel_GBs": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
 
+    uneven = None if a.no_uneven else uneven_groups(B, psi, a.uneven_steps, autotune=not a.no_autotune)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(psi, a.cpu_seconds)
@@ -178,6 +226,7 @@ def main():
             "residual": rep_grad["residual"], "method": "preconditioned imaginary-time gradient flow, FP64",
             "reference_rule_ms": rep_ref["elapsed_ms"], "reference_rule_iterations": rep_ref["iterations"],
             "reference_rule_residual": rep_ref["residual"]},
+        "uneven_groups": uneven,
         "final_observables": {k: obs[k] for k in ("t", "N_total", "visibility", "visibility_grid")},
         "n_samples": int(samples.shape[0]),
     }
