@@ -1,7 +1,7 @@
 """Learned ground-state start: parameter sampling, network inputs, training and
 the time-to-residual comparison against the Thomas-Fermi start.
 
-    python -m bec_amd.surrogate [--train 2000 --test 200 --epochs 60 --seed 7 --out FILE]
+    python -m bec_amd.surrogate [--train 2000 --test 200 --epochs 150 --seed 7 --out FILE]
 
 One run generates the two-component ground states with the library's own
 solver (bec_ground_states_two_component, FP64, gradient flow), trains the
@@ -17,7 +17,8 @@ a22 = 95.0; half of the atoms in each component; lattice 128 x 16 x 16.
 Network inputs per cell (16 channels): the one-component Thomas-Fermi
 amplitude sqrt(max(mu - V, 0) / mu), V / mu, the cell coordinates scaled to
 [-1, 1] (x, y, z), the four parameters scaled to [-1, 1], and zeros.  Outputs:
-the two component amplitudes in units of sqrt(mu / g11).  The lattice box
+the sum and the weighted difference of the two component amplitudes, in units
+of sqrt(mu / g11) (to_net / from_net).  The lattice box
 scales with mu (xmax = 1.2 sqrt(2 mu)), so these targets are O(1) over the
 whole parameter range.
 """
@@ -78,20 +79,47 @@ def features(c, theta):
     return f.reshape(-1, C_IN)
 
 
+# The network predicts the sum s = (a + b)/sqrt 2 and the difference d = (a - b)/sqrt 2 of
+# the component amplitudes, with d scaled by SPIN_WEIGHT.  The squared-error loss then
+# weighs errors in the relative density SPIN_WEIGHT^2 times more: that mode is soft near
+# the miscibility threshold, so its errors are the slow ones for the refining solver,
+# while errors of the total density are removed in a few iterations.
+SPIN_WEIGHT = 3.0
+
+
 def amp_unit(c):
     return math.sqrt(c.mu / c.g11)
 
 
+def to_net(a, b):
+    r = 1.0 / math.sqrt(2.0)
+    return np.stack([(a + b) * r, (a - b) * r * SPIN_WEIGHT], axis=-1)
+
+
+def from_net(y):
+    r = 1.0 / math.sqrt(2.0)
+    s, d = y[..., 0], y[..., 1] / SPIN_WEIGHT
+    return (s + d) * r, (s - d) * r
+
+
 def targets(c, state):
-    """state [2][nz][ny][nx] complex (x-space ground state) -> [cells][2] float32."""
+    """state [2][nz][ny][nx] complex (x-space ground state) -> [cells][2] float32 network targets."""
     u = amp_unit(c)
-    return np.stack([state[0].real.ravel() / u, state[1].real.ravel() / u], axis=-1).astype(np.float32)
+    return to_net(state[0].real.ravel() / u, state[1].real.ravel() / u).astype(np.float32)
 
 
 def guesses(c, pred):
     """Network output [cells][2] -> solver guess [2][cells] complex."""
     u = amp_unit(c)
-    return (np.asarray(pred, dtype=np.float64).T * u).astype(np.complex128)
+    a, b = from_net(np.asarray(pred, dtype=np.float64))
+    return (np.stack([a, b]) * u).astype(np.complex128)
+
+
+def guess_error(pred, target):
+    """Relative L2 error of the predicted component amplitudes."""
+    pa, pb = from_net(np.asarray(pred, dtype=np.float64))
+    ta, tb = from_net(np.asarray(target, dtype=np.float64))
+    return float(np.sqrt((np.sum((pa - ta) ** 2) + np.sum((pb - tb) ** 2)) / (np.sum(ta ** 2) + np.sum(tb ** 2))))
 
 
 def solve(calcs, guess=None, tol=1e-6, max_iter=20000):
@@ -186,7 +214,7 @@ def evaluate(net, thetas, X, Y, tol=1e-6, log=print):
                      "tf_ms": r_tf[0]["elapsed_ms"], "tf_residual": r_tf[0]["residual"],
                      "nn_iters": r_nn[0]["iterations"], "nn_ms": r_nn[0]["elapsed_ms"],
                      "nn_residual": r_nn[0]["residual"], "inference_ms": t_inf,
-                     "guess_rel_l2": rel_l2(pred, Y[i]), "converged": bool(st_tf == 0 and st_nn == 0)})
+                     "guess_rel_l2": guess_error(pred, Y[i]), "converged": bool(st_tf == 0 and st_nn == 0)})
         if (i + 1) % 50 == 0:
             log(f"evaluated {i + 1}/{len(thetas)}")
     return rows
@@ -216,7 +244,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=200)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)

@@ -98,6 +98,31 @@ def uneven_groups(B, psi_a, steps, autotune=True):
             "ground_state_256x32x32": {"iterations": rep_b["iterations"], "residual": rep_b["residual"]}}
 
 
+SURROGATE_SUMMARY = os.path.join(ROOT, "profiles", "surrogate_groundstate_200heldout.json")
+
+
+def two_component_ground_state(B):
+    """Two-component ground state (N/2 per component) at the headline parameters, gradient flow
+    from the Thomas-Fermi start to residual 1e-6, FP64 (second solve timed)."""
+    c = B.derive(B.default_model(**config1(ne=1)))
+    for _ in range(2):
+        out, reps, st = B.ground_states_two_component([c], tol=1e-6, max_iter=20000, precision=B.FP64)
+    r = reps[0]
+    return {"time_to_residual_1e-6_ms": r["elapsed_ms"], "iterations": r["iterations"], "residual": r["residual"],
+            "converged": bool(st == 0), "start": "Thomas-Fermi",
+            "method": "preconditioned imaginary-time gradient flow, FP64, coupled components"}
+
+
+def surrogate_summary():
+    """The committed result of `python -m bec_amd.surrogate` (not measured by this run)."""
+    if not os.path.exists(SURROGATE_SUMMARY):
+        return None
+    with open(SURROGATE_SUMMARY) as f:
+        d = json.load(f)
+    return {"source": os.path.relpath(SURROGATE_SUMMARY, ROOT), "command": d.get("command"),
+            "summary": d.get("summary")}
+
+
 def cpu_baseline(psi, seconds):
     """FP64 oracle split-step (same algorithm and seeds) timed on this host's cores."""
     from oracle import oracle as O
@@ -203,6 +228,7 @@ def main():
             "copy_kernel_GBs": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
 
     uneven = None if a.no_uneven else uneven_groups(B, psi, a.uneven_steps, autotune=not a.no_autotune)
+    gs2 = two_component_ground_state(B)
 
     cpu = None
     if not a.no_cpu:
@@ -226,6 +252,8 @@ def main():
             "residual": rep_grad["residual"], "method": "preconditioned imaginary-time gradient flow, FP64",
             "reference_rule_ms": rep_ref["elapsed_ms"], "reference_rule_iterations": rep_ref["iterations"],
             "reference_rule_residual": rep_ref["residual"]},
+        "ground_state_two_component": gs2,
+        "surrogate_start": surrogate_summary(),
         "uneven_groups": uneven,
         "final_observables": {k: obs[k] for k in ("t", "N_total", "visibility", "visibility_grid")},
         "n_samples": int(samples.shape[0]),

@@ -42,6 +42,7 @@ def test_features_and_targets_layout(bec):
     t = S.targets(c, out[0])
     g = S.guesses(c, t)
     assert np.abs(g - out[0].reshape(2, -1).real).max() < 1e-5 * np.abs(out[0]).max()
+    assert S.guess_error(t, t) == 0.0
     # the Thomas-Fermi channel peaks at the trap centre
     tf = f[:, 0].reshape(128, 16, 16)
     assert np.unravel_index(np.argmax(tf), tf.shape)[0] in (63, 64)
