@@ -44,21 +44,25 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
                     }
         return n;
     }
+    // the z pass is instantiated only for LDS exchange tiles up to 64 KiB and
+    // workgroups up to 1024 threads (launch_z)
+    auto fits = [&](int pw, int r1, int half) {
+        return (long)nz * pw * (half ? 4 : 8) <= 65536 && (nz / r1) * pw <= 1024 && pw <= nx * nx;
+    };
     const int r1s[2] = {default_r1(nz), nz == 64 ? 16 : 32};
     for (int pw : {32, 64})
         for (int r1 : r1s)
             for (int nt = 0; nt < 2; ++nt)
                 for (int half = 0; half < 2; ++half)
-                    for (int xcd = 0; xcd < xcds; ++xcd) {
-                        const long lds = (long)nz * pw * (half ? 4 : 8);
-                        if (lds > 65536 || (nz / r1) * pw > 1024) continue;
-                        add(Schedule{pw, r1, 1, nt, xcd, 1, half});
-                    }
+                    for (int xcd = 0; xcd < xcds; ++xcd)
+                        if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcd, 1, half});
     // the xy occupancy target on the streaming, XCD-aware family
     for (int pw : {32, 64})
-        for (int half = 0; half < 2; ++half) add(Schedule{pw, default_r1(nz), 1, 1, xcds - 1, 5, half});
+        for (int half = 0; half < 2; ++half)
+            if (fits(pw, default_r1(nz), half)) add(Schedule{pw, default_r1(nz), 1, 1, xcds - 1, 5, half});
     // the unsplit z pass for reference
-    add(Schedule{default_pw(prec, nx, nz), default_r1(nz), 0, 1, xcds - 1, 1, 0});
+    if (fits(default_pw(prec, nx, nz), default_r1(nz), 0))
+        add(Schedule{default_pw(prec, nx, nz), default_r1(nz), 0, 1, xcds - 1, 1, 0});
     return n;
 }
 

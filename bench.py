@@ -26,6 +26,11 @@ HBM_PEAK = 8.0e12
 SEED = 0x5EED
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def config1(**kw):
     d = dict(N=30000, nvx=16, nvy=16, nvz=128, ne=64, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
              detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
@@ -179,6 +184,7 @@ def main():
         psi, rep_grad, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
                                                     precision=B.FP64, max_iter=5000)
 
+    log("ground states done")
     # headline workload
     c = B.derive(B.default_model(**config1(ne=a.traj)))
     s = B.Solver(c, precision=B.FP32)
@@ -196,6 +202,7 @@ def main():
     ms_cfg = s.time_passes(iters=200)
     s.close()
 
+    log(f"headline run done: {value:.4g} trajectory-steps/s")
     # roofline on a state beyond the Infinity Cache
     cb = B.derive(B.default_model(**config1(ne=a.roofline_traj)))
     sb = B.Solver(cb, precision=B.FP32)
@@ -227,8 +234,11 @@ def main():
             "passes_per_step": 2, "schedule": sched_big,
             "copy_kernel_GBs": 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9}
 
+    log("roofline leg done")
     uneven = None if a.no_uneven else uneven_groups(B, psi, a.uneven_steps, autotune=not a.no_autotune)
+    log("uneven-groups leg done")
     gs2 = two_component_ground_state(B)
+    log("two-component ground state done")
 
     cpu = None
     if not a.no_cpu:

@@ -142,6 +142,21 @@ def test_autotune_keeps_state_and_results(bec, oracle, gs):
         assert rel_l2(o, outs[0]) < 1e-6
 
 
+@pytest.mark.parametrize("nxy,nz", [(16, 64), (16, 128), (16, 256), (32, 64), (32, 128), (32, 256)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_autotune_every_supported_lattice(bec, nxy, nz, prec):
+    """Every candidate schedule of every supported lattice launches (the candidate
+    list respects the z pass's LDS and workgroup limits)."""
+    c = gpu_calc(bec, N=30000, nvx=nxy, nvy=nxy, nvz=nz, ne=8)
+    with bec.Solver(c, precision=prec) as s:
+        psi = np.zeros((nz, nxy, nxy), dtype=complex)
+        psi[nz // 2, nxy // 2, nxy // 2] = 1.0
+        s.init_evolution(psi, SEED, equilibrate=False)
+        best, res = s.autotune(steps=1)
+        assert res and all(r["ms_per_step"] > 0 for r in res)
+        assert best in res or any(abs(r["ms_per_step"] - best["ms_per_step"]) < 1e-12 for r in res)
+
+
 def test_uneven_groups_match_single_group_runs(bec, oracle, gs):
     """Two grid groups (128x16x16 and 256x32x32) stepped together -- the graph replays
     one branch per group -- give the same states as each group stepped alone."""
