@@ -206,6 +206,35 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
         const T yy = (T)(-p.ymax) + (T)p.dy * (T)y;
         const T zz = (T)(-p.zmax) + (T)p.dz * (T)z;
         const T vyz = yy * yy + zz * zz * (T)p.inv_lambda2;
+        if constexpr (IT1) {
+            // itmax = 1: d = exp(p(psi0) dt/2) and the output psi0 d^2, i.e. the
+            // phase 2 phi = Im(p) dt and the loss exponent y = Re(p) dt, with dt
+            // folded into the (uniform) coefficients
+            const double dt = p.dt;
+            const T g11d = (T)(p.g11 * dt), g12d = (T)(p.g12 * dt), g22d = (T)(p.g22 * dt);
+            const T n111 = (T)(-0.5 * p.l111 * dt), n12 = (T)(-0.5 * p.l12 * dt), n22 = (T)(-0.5 * p.l22 * dt);
+            const T detd = (T)(p.detuning * dt);
+            static_for<0, N>([&](auto X_) {
+                constexpr int x = decltype(X_)::value;
+                const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
+                const T Vd = half_dt * fma_t(xx, xx, vyz);     // V dt
+                const cpx<T> a0 = r[0][x], b0 = r[1][x];
+                const T na = cnorm<T>(a0), nb = cnorm<T>(b0);
+                const T pha = fma_t(g11d, na, fma_t(g12d, nb, Vd));
+                const T phb = fma_t(g22d, nb, fma_t(g12d, na, Vd - detd));
+                const T ya = fma_t(n111, na * na, n12 * nb);
+                const T yb = fma_t(n22, nb, n12 * na);
+                T sa, ca, sb, cb;
+                sincos2_t(pha, &sa, &ca);
+                sincos2_t(phb, &sb, &cb);
+                const T ta = expm1_small_t(ya), tb = expm1_small_t(yb);
+                const cpx<T> va = cscale<T>(cmul<T>(a0, mk<T>(ca, sa)), scale);
+                const cpx<T> vb = cscale<T>(cmul<T>(b0, mk<T>(cb, sb)), scale);
+                r[0][x] = mk<T>(fma_t(va.x, ta, va.x), fma_t(va.y, ta, va.y));
+                r[1][x] = mk<T>(fma_t(vb.x, tb, vb.x), fma_t(vb.y, tb, vb.y));
+            });
+            return;
+        }
         static_for<0, N>([&](auto X_) {
             constexpr int x = decltype(X_)::value;
             const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
@@ -230,14 +259,10 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
                 ta = expm1_t((T)2 * pa_re * half_dt);
                 tb = expm1_t((T)2 * pb_re * half_dt);
             };
-            if constexpr (IT1) {
-                factors(a0, b0);
-            } else {
-                for (int it = 0; it < itmax; ++it) {
-                    // midpoint: psi0 d = psi0 e^{i phi} sqrt(1 + t)
-                    factors(cscale<T>(cmul<T>(a0, ea), sqrt_t((T)1 + ta)),
-                            cscale<T>(cmul<T>(b0, eb), sqrt_t((T)1 + tb)));
-                }
+            for (int it = 0; it < itmax; ++it) {
+                // midpoint: psi0 d = psi0 e^{i phi} sqrt(1 + t)
+                factors(cscale<T>(cmul<T>(a0, ea), sqrt_t((T)1 + ta)),
+                        cscale<T>(cmul<T>(b0, eb), sqrt_t((T)1 + tb)));
             }
             const cpx<T> va = cscale<T>(cmul<T>(a0, cmul<T>(ea, ea)), scale);
             const cpx<T> vb = cscale<T>(cmul<T>(b0, cmul<T>(eb, eb)), scale);
@@ -245,13 +270,34 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
             r[1][x] = mk<T>(fma_t(vb.x, tb, vb.x), fma_t(vb.y, tb, vb.y));
         });
     }
-    // expm1: a degree-5 Taylor polynomial for |y| < 1/64 (truncation error
-    // < y^6/720 < 2e-14 relative), else exp(y) - 1
+    // sin and cos of the doubled phase (IT1 path)
+    static __device__ __forceinline__ void sincos2_t(T x, T *s, T *c)
+    {
+        if constexpr (sizeof(T) == 4) fast_sincos_pi(x, s, c);
+        else sincos(x, s, c);
+    }
+    // expm1 for the IT1 path: the polynomial; only when a lane of the wave has
+    // |y| >= 1/64 (a loss of > 1.5% in one step) does the wave take exp(y) - 1
+    static __device__ __forceinline__ T expm1_small_t(T y)
+    {
+        if constexpr (sizeof(T) == 4) {
+            float p = fmaf(y, 1.0f / 24.0f, 1.0f / 6.0f);
+            p = fmaf(y, p, 0.5f);
+            p = fmaf(y, p, 1.0f);
+            float t = y * p;
+            const bool big = fabsf(y) >= 0.015625f;
+            if (__any(big)) t = big ? __expf(y) - 1.0f : t;
+            return t;
+        } else {
+            return expm1(y);
+        }
+    }
+    // expm1: a degree-4 Taylor polynomial for |y| < 1/64 (truncation error
+    // < y^4/120 < 5e-10 relative), else exp(y) - 1
     static __device__ __forceinline__ T expm1_t(T y)
     {
         if constexpr (sizeof(T) == 4) {
-            float p = fmaf(y, 1.0f / 120.0f, 1.0f / 24.0f);
-            p = fmaf(y, p, 1.0f / 6.0f);
+            float p = fmaf(y, 1.0f / 24.0f, 1.0f / 6.0f);
             p = fmaf(y, p, 0.5f);
             p = fmaf(y, p, 1.0f);
             const float big = __expf(y) - 1.0f;

@@ -12,10 +12,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS = r"""
 #include <stdio.h>
 #include "fastmath.hpp"
-int main() {
+int main(int argc, char **argv) {
     float x; float s, c;
+    const bool by_pi = argc > 1;
     while (fread(&x, sizeof(float), 1, stdin) == 1) {
-        bec::fast_sincosf(x, &s, &c);
+        if (by_pi) bec::fast_sincos_pi(x, &s, &c);
+        else bec::fast_sincosf(x, &s, &c);
         fwrite(&s, sizeof(float), 1, stdout);
         fwrite(&c, sizeof(float), 1, stdout);
     }
@@ -36,18 +38,20 @@ def harness():
     return exe
 
 
-def _run(exe, x):
+def _run(exe, x, by_pi=False):
     x = np.ascontiguousarray(x, dtype=np.float32)
-    out = subprocess.run([exe], input=x.tobytes(), stdout=subprocess.PIPE, check=True).stdout
+    out = subprocess.run([exe] + (["pi"] if by_pi else []), input=x.tobytes(), stdout=subprocess.PIPE,
+                         check=True).stdout
     sc = np.frombuffer(out, dtype=np.float32).reshape(-1, 2)
     return sc[:, 0], sc[:, 1]
 
 
-def test_fast_sincos_accuracy(harness):
+@pytest.mark.parametrize("by_pi", [False, True], ids=["halfpi_reduction", "pi_reduction"])
+def test_fast_sincos_accuracy(harness, by_pi):
     r = np.random.default_rng(0)
     x = np.concatenate([np.linspace(-8, 8, 200001), r.uniform(-200, 200, 200000), r.uniform(-1e-3, 1e-3, 1000),
                         np.array([0.0, np.pi / 4, np.pi / 2, np.pi, 3 * np.pi / 2, 1e4])]).astype(np.float32)
-    s, c = _run(harness, x)
+    s, c = _run(harness, x, by_pi)
     xd = x.astype(np.float64)
     es = np.abs(s - np.sin(xd))
     ec = np.abs(c - np.cos(xd))
