@@ -80,7 +80,7 @@ class Gs2Report(C.Structure):
 class Obs(C.Structure):
     _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
                 ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
-                ("N1", C.c_double), ("N2", C.c_double)]
+                ("N1", C.c_double), ("N2", C.c_double), ("Na_stderr", C.c_double), ("Nb_stderr", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -696,6 +696,26 @@ static void obs_from_sums(const bec_calc_params &c, const double *sums, int ne, 
     o->N2 = N2 * dV;
     o->visibility = 2.0 * sqrt(sre * sre + sim * sim) / (N1 + N2);
     o->phase = atan2(sim, sre);
+    // spread of the per-trajectory populations after the virtual pulse (alpha = 0)
+    // (two passes over the trajectories: mean, then squared deviations)
+    const double vac1 = (double)c.cells * c.V / (2.0 * dV);
+    auto pop = [&](int e, int which) {
+        const double *s4 = sums + 4 * e;
+        return (0.5 * (s4[0] + s4[1]) + (which ? -s4[3] : s4[3]) - vac1) * dV;
+    };
+    double m1 = 0, m2 = 0, q1 = 0, q2 = 0;
+    for (int e = 0; e < ne; ++e) {
+        m1 += pop(e, 0);
+        m2 += pop(e, 1);
+    }
+    m1 /= ne;
+    m2 /= ne;
+    for (int e = 0; e < ne; ++e) {
+        q1 += (pop(e, 0) - m1) * (pop(e, 0) - m1);
+        q2 += (pop(e, 1) - m2) * (pop(e, 1) - m2);
+    }
+    o->Na_stderr = ne > 1 ? sqrt(q1 / ne / (ne - 1)) : 0.0;
+    o->Nb_stderr = ne > 1 ? sqrt(q2 / ne / (ne - 1)) : 0.0;
 }
 
 extern "C" int bec_observables(bec_ctx *ctx, int group, bec_obs *out)

@@ -81,15 +81,18 @@ std::string list(const std::vector<double> &v)
     return s + "]";
 }
 
+// The xy schema of the reference's data/*.yaml; `yerrors` (standard errors of
+// the ensemble means) is an extra key written only when errors are given.
 int write_series(const std::string &path, const std::vector<double> &x, const std::vector<double> &y,
                  const char *xname, const char *yname, const std::string &name, const std::string &desc,
-                 double ymin, double ymax)
+                 double ymin, double ymax, const std::vector<double> *yerr = nullptr)
 {
     FILE *f = fopen(path.c_str(), "w");
     if (!f) return 1;
     fprintf(f, "xarray: '%s'\n", list(x).c_str());
     fprintf(f, "xname: %s\n", xname);
     fprintf(f, "yarray: '%s'\n", list(y).c_str());
+    if (yerr) fprintf(f, "yerrors: '%s'\n", list(*yerr).c_str());
     fprintf(f, "ymax: %.17g\nymin: %.17g\n", ymax, ymin);
     fprintf(f, "yname: %s\n", yname);
     fprintf(f, "description: '%s'\n", desc.c_str());
@@ -182,7 +185,7 @@ int main(int argc, char **argv)
 
     const int steps = (int)num(kv, "steps", 15000);
     const int every = (int)num(kv, "sample_every", 250);
-    std::vector<double> t, vis, visg, ntot, na, nb, phase;
+    std::vector<double> t, vis, visg, ntot, na, nb, phase, na_err, nb_err;
     bec_obs ob;
     CHECK(bec_observables(ctx, 0, &ob));
     auto record = [&](const bec_obs &o2) {
@@ -192,6 +195,8 @@ int main(int argc, char **argv)
         ntot.push_back(o2.N_total);
         na.push_back(o2.Na / c.ne);
         nb.push_back(o2.Nb / c.ne);
+        na_err.push_back(o2.Na_stderr);
+        nb_err.push_back(o2.Nb_stderr);
         phase.push_back(o2.phase);
     };
     record(ob);
@@ -211,8 +216,8 @@ int main(int argc, char **argv)
     w |= write_series(prefix + "_visibility.yaml", t, visg, "Time, s", "Visibility", name, desc, 0.0, 1.0);
     w |= write_series(prefix + "_visibility_analytic.yaml", t, vis, "Time, s", "Visibility", name, desc, 0.0, 1.0);
     w |= write_series(prefix + "_N.yaml", t, ntot, "Time, s", "N", name, desc, 0.0, (double)c.N);
-    w |= write_series(prefix + "_Na.yaml", t, na, "Time, s", "N", name, desc, 0.0, (double)c.N);
-    w |= write_series(prefix + "_Nb.yaml", t, nb, "Time, s", "N", name, desc, 0.0, (double)c.N);
+    w |= write_series(prefix + "_Na.yaml", t, na, "Time, s", "N", name, desc, 0.0, (double)c.N, &na_err);
+    w |= write_series(prefix + "_Nb.yaml", t, nb, "Time, s", "N", name, desc, 0.0, (double)c.N, &nb_err);
     w |= write_series(prefix + "_phase.yaml", t, phase, "Time, s", "Phase", name, desc, -M_PI, M_PI);
     bec_destroy(ctx);
     if (w) {

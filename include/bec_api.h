@@ -203,6 +203,8 @@ typedef struct bec_obs {
     double visibility;          /* 2 |<psi1|psi2>| / (N1 + N2), vacuum-subtracted */
     double phase;               /* arg sum conj(a) b: mean relative phase */
     double N1, N2;              /* vacuum-subtracted populations without the virtual pulse, dV units */
+    double Na_stderr, Nb_stderr; /* standard error of the ensemble mean of the per-trajectory Na, Nb
+                                    (sqrt(var / (ne - 1)), the error bars of 2d/wigner.cpp:438-443); 0 for ne = 1 */
 } bec_obs;
 
 int bec_observables(bec_ctx *ctx, int group, bec_obs *out);

@@ -73,6 +73,11 @@ def test_samples_and_observables(bec, oracle, gs, prec):
     assert abs(obs["Nb"] - n2 * dV) / abs(n2 * dV) < 1e-9
     assert abs(obs["N_total"] - (n1 + n2) * dV / oc.ne) < 1e-9 * abs(obs["N_total"])
     assert abs(obs["visibility_grid"] - oracle.visibility_grid(oc, ax, bx)) < 1e-9
+    # error bars: standard error of the ensemble mean of the per-trajectory populations
+    per = np.array([oracle.rotate_sums(oc, ax[j], bx[j], 0.0) for j in range(oc.ne)]) * dV
+    se = per.std(axis=0, ddof=1) / np.sqrt(oc.ne)
+    assert obs["Na_stderr"] == pytest.approx(se[0], rel=1e-7)
+    assert obs["Nb_stderr"] == pytest.approx(se[1], rel=1e-7)
     # projections vs per-cell ensemble sums
     da, db = oracle.cell_density_sums(oc, ax, bx, 0.0)
     assert rel_l2(proj["xy_a"], da.sum(axis=0)) < 1e-9
