@@ -77,6 +77,25 @@ class Gs2Report(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Params2D(C.Structure):
+    """bec_2d_params: the file-scope constants of programs/bec_wigner_2d.cpp/wigner.cpp:106-136."""
+    _fields_ = [("nvx", C.c_int), ("nvy", C.c_int), ("xmax", C.c_double), ("ymax", C.c_double),
+                ("g11", C.c_double), ("g12", C.c_double), ("g22", C.c_double),
+                ("E", C.c_double), ("mu", C.c_double), ("Va", C.c_double), ("Vb", C.c_double),
+                ("dtG", C.c_double), ("ntG", C.c_int), ("itmax", C.c_int),
+                ("dtW", C.c_double), ("ntW", C.c_int), ("n_meas", C.c_int), ("ne", C.c_int),
+                ("n0", C.c_double), ("seed", C.c_uint64), ("noise_dev", C.c_double),
+                ("gs_rule", C.c_int), ("gs_tol", C.c_double), ("gs_max_iter", C.c_int)]
+
+
+class Report2D(C.Structure):
+    _fields_ = [("N0", C.c_double), ("gs_iterations", C.c_int), ("gs_residual", C.c_double),
+                ("gs_mu", C.c_double), ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Obs(C.Structure):
     _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
                 ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
@@ -113,6 +132,7 @@ EXPORTS = {
     "bec_ground_states_two_component": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, _P, C.POINTER(GsOptions),
                                                   _P, _P, C.POINTER(Gs2Report)]),
     "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
+    "bec_run_2d": (C.c_int, [C.POINTER(Params2D), _P, _P, _P, _P, _P, C.POINTER(Report2D)]),
     "bec_init_evolution": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int]),
     "bec_propagate": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _P]),
     "bec_propagate_graph": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
@@ -250,6 +270,32 @@ def ground_states_two_component(params, frac_a=None, guesses=None, **opt):
     if st not in (OK, ERR_NOT_CONVERGED):
         check(st, "bec_ground_states_two_component")
     return out, [r.as_dict() for r in reps], st
+
+
+def params2d(**kw):
+    """The 2D program's constants (programs/bec_wigner_2d.cpp/wigner.cpp:106-136) on a 64 x 64 lattice
+    (its 128 x 128 does not fit one workgroup); keyword arguments override."""
+    d = dict(nvx=64, nvy=64, xmax=20.0, ymax=20.0, g11=6.18, g12=6.03, g22=5.88, E=2.5, mu=26.8, Va=0.2, Vb=0.5,
+             dtG=1.0 / 31.0, ntG=32, itmax=3, dtW=0.01, ntW=200, n_meas=1, ne=32, n0=1.0, seed=0x5EED,
+             noise_dev=None, gs_rule=0, gs_tol=1e-8, gs_max_iter=100000)
+    d.update(kw)
+    if d["noise_dev"] is None:
+        d["noise_dev"] = 1.0 / np.sqrt((2.0 * d["xmax"] / (d["nvx"] - 1)) * (2.0 * d["ymax"] / (d["nvy"] - 1)))
+    return Params2D(**d)
+
+
+def run_2d(p, want_avn=True):
+    """The 2D program on the GPU: ground state, then ne trajectories (bec_run_2d)."""
+    nm = p.n_meas
+    m = np.zeros(2 * nm)
+    var = np.zeros(2 * nm)
+    vis = np.zeros(nm)
+    gs = np.empty((p.nvy, p.nvx), dtype=np.complex128)
+    avn = np.empty((p.nvy, p.nvx)) if want_avn else None
+    rep = Report2D()
+    check(lib().bec_run_2d(C.byref(p), _ptr(m), _ptr(var), _ptr(vis), _ptr(gs),
+                           _ptr(avn) if want_avn else None, C.byref(rep)), "bec_run_2d")
+    return dict(m=m.reshape(2, nm), var=var.reshape(2, nm), vis=vis, gs=gs, avn=avn, report=rep.as_dict())
 
 
 class Solver:

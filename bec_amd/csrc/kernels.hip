@@ -1,6 +1,7 @@
 // Kernel launchers for gfx950: dispatch from runtime lattice shape and
 // precision to the compile-time pass templates of passes.hpp.
 #include "launch.hpp"
+#include "philox.hpp"
 
 namespace bec {
 
@@ -231,28 +232,8 @@ hipError_t run_reduce_partials(const double *partial, int per_traj, int ntraj, i
 }
 
 // ------------------------------------------------------------ noise init
-// Philox4x32-10 (Salmon et al., SC'11) with counter {cell, traj, 0, 0} and
-// key = seed; Box-Muller of fillWithNormalDistribution (wigner.cu:15-34,
-// dev 0.5) evaluated in FP64; initializeEnsembles (wigner_kernel.cu:295-317).
-
-__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r > 0) {
-            k0 += 0x9E3779B9u;
-            k1 += 0xBB67AE85u;
-        }
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = (uint32_t)p1;
-        c[2] = n2;
-        c[3] = (uint32_t)p0;
-    }
-}
+// Philox noise (philox.hpp) with counter {cell, traj, 0, 0} and key = seed;
+// initializeEnsembles (wigner_kernel.cu:295-317).
 
 template <typename T>
 __global__ void noise_init_kernel(const cpx<T> *__restrict__ gs, cpx<T> *__restrict__ a, cpx<T> *__restrict__ b,
@@ -263,19 +244,8 @@ __global__ void noise_init_kernel(const cpx<T> *__restrict__ gs, cpx<T> *__restr
     if (i >= n) return;
     const long cell = i % cells;
     const long traj = i / cells;
-    uint32_t c[4] = {(uint32_t)cell, (uint32_t)traj, 0u, 0u};
-    philox4x32_10(c, k0, k1);
     double eta[4];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        const double u1 = ((double)c[2 * p] + 1.0) * (1.0 / 4294967296.0);
-        const double u2 = ((double)c[2 * p + 1] + 1.0) * (1.0 / 4294967296.0);
-        const double rad = 0.5 * sqrt(-2.0 * log(u1));
-        double sn, cs;
-        sincospi(2.0 * u2, &sn, &cs);
-        eta[2 * p] = rad * cs;
-        eta[2 * p + 1] = rad * sn;
-    }
+    noise4((uint32_t)cell, (uint32_t)traj, 0u, k0, k1, eta);
     const cpx<T> g = gs[cell];
     a[i] = mk<T>((T)((double)g.x + Va * eta[0] * coeff), (T)((double)g.y + Va * eta[1] * coeff));
     b[i] = mk<T>((T)(Vb * eta[2] * coeff), (T)(Vb * eta[3] * coeff));

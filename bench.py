@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -104,6 +106,36 @@ def uneven_groups(B, psi_a, steps, autotune=True):
 
 
 SURROGATE_SUMMARY = os.path.join(ROOT, "profiles", "surrogate_groundstate_200heldout.json")
+
+
+def program_2d(B, with_cpu):
+    """The 2D program (config 0: 64 x 64, complex128, the program's constants, its fixed 31-step
+    ground state, pi/2 pulse, 2000 real-time steps, one trajectory without noise), on the GPU
+    (one workgroup, state in LDS) and, for comparison, the FP64 oracle on the host cores; plus
+    the same evolution for 256 noisy trajectories on the GPU (one workgroup each)."""
+    kw = dict(nvx=64, nvy=64, ne=1, Va=0.0, Vb=0.0, ntW=2000, n_meas=1)
+    B.run_2d(B.params2d(**dict(kw, ntW=10)))                 # first-call setup
+    t0 = time.perf_counter()
+    one = B.run_2d(B.params2d(**kw))
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens = B.run_2d(B.params2d(**dict(kw, ne=256, Va=0.2, Vb=0.5)), want_avn=False)
+    t_ens = time.perf_counter() - t0
+    out = {"workload": "64x64 complex128, 31 ground-state steps, pi/2 pulse, 2000 steps; 1 trajectory "
+                       "(and 256 with the program's noise)",
+           "gpu_ms_1traj": t_gpu * 1e3, "gpu_device_ms_1traj": one["report"]["elapsed_ms"],
+           "gpu_ms_256traj": t_ens * 1e3, "traj_steps_per_s_256": 256 * 2000 / t_ens,
+           "N1_N2_1traj": [float(one["m"][0, 0]), float(one["m"][1, 0])]}
+    if with_cpu:
+        from oracle import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        ref = O.run2d(O.params2d(**kw))
+        t_cpu = time.perf_counter() - t0
+        out.update(cpu_ms_1traj=t_cpu * 1e3, cpu_threads=threads, cpu_kind="port",
+                   max_rel_diff_N=float(np.max(np.abs(one["m"] - ref["m"]) / np.abs(ref["m"]).max())))
+    return out
 
 
 def two_component_ground_state(B):
@@ -239,6 +271,8 @@ def main():
     log("uneven-groups leg done")
     gs2 = two_component_ground_state(B)
     log("two-component ground state done")
+    p2d = program_2d(B, not a.no_cpu)
+    log("2D program done")
 
     cpu = None
     if not a.no_cpu:
@@ -265,6 +299,7 @@ def main():
         "ground_state_two_component": gs2,
         "surrogate_start": surrogate_summary(),
         "uneven_groups": uneven,
+        "program_2d": p2d,
         "final_observables": {k: obs[k] for k in ("t", "N_total", "visibility", "visibility_grid")},
         "n_samples": int(samples.shape[0]),
     }

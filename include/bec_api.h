@@ -174,6 +174,50 @@ int bec_ground_states_two_component(const bec_calc_params *params, int batch, co
 /* Residual ||H psi - mu psi||/(|mu| ||psi||) and mu of an x-space state, FP64 on the GPU. */
 int bec_state_residual(bec_ctx *ctx, int group, const double *psi_x, double *residual, double *mu);
 
+/* ------------------------------------------------------- the 2D program
+ * programs/bec_wigner_2d.cpp/wigner.cpp on the GPU, FP64: calculate_steady_state
+ * (:196-275) then main_loop (:278-477) for ne trajectories, one workgroup per
+ * trajectory with its state resident in LDS.  Field meanings are the program's
+ * file-scope constants (:106-136) plus the extensions of the measurement loop:
+ * n_meas samples after the pulse (the program takes 1), and gs_rule = 1 to stop
+ * the ground state at residual < gs_tol instead of after ntG - 1 steps.  The
+ * noise is Philox (counter {cell, trajectory, 0, 0}, key seed) with deviation
+ * noise_dev (the program: 1/sqrt(dx dy)).  Lattices: nvx, nvy powers of two
+ * from 4 to 64 (the program's 128 x 128 does not fit a workgroup). */
+typedef struct bec_2d_params {
+    int nvx, nvy;
+    double xmax, ymax;
+    double g11, g12, g22, E, mu;
+    double Va, Vb;
+    double dtG;
+    int ntG, itmax;
+    double dtW;
+    int ntW;          /* equilibration steps, the pulse step included */
+    int n_meas;
+    int ne;
+    double n0;        /* uniform start of the ground state */
+    uint64_t seed;
+    double noise_dev;
+    int gs_rule;
+    double gs_tol;
+    int gs_max_iter;
+} bec_2d_params;
+
+typedef struct bec_2d_report {
+    double N0;            /* atoms of the ground state */
+    int gs_iterations;
+    double gs_residual;   /* ||H psi - mu psi|| / (|mu| ||psi||) of the ground state */
+    double gs_mu;
+    double elapsed_ms;    /* device time of ground state + ensemble */
+} bec_2d_report;
+
+/* m, var: [2][n_meas] ensemble means of N1, N2 and their standard errors
+ * (wigner.cpp:431-443); vis: [n_meas] mean visibility; gs_x: [cells] complex
+ * ground state (may be NULL); avn: [cells] mean x-space density minus the
+ * vacuum term after the last step (may be NULL). */
+int bec_run_2d(const bec_2d_params *p, double *m, double *var, double *vis, double *gs_x, double *avn,
+               bec_2d_report *rep);
+
 /* ---------------------------------------------------------- evolution */
 /* initEvolution (wigner.cu:202-227): ground state + Philox vacuum noise of
  * variance 1/(2 dV) per mode (initializeEnsembles, wigner_kernel.cu:295-317),

@@ -102,6 +102,12 @@ def test_invalid_arguments_rejected_on_cpu():
     bad_frac = (C.c_double * 1)(1.0)
     assert L.bec_ground_states_two_component(C.byref(cc), 1, bad_frac, None, C.byref(o), None, None, reps2) == \
         api.ERR_INVALID
+    # the 2D program: lattices beyond one workgroup's LDS, bad rules
+    m2, v2 = (C.c_double * 2)(), (C.c_double * 2)()
+    p2 = bec_amd.params2d(nvx=128, nvy=128)
+    assert L.bec_run_2d(C.byref(p2), m2, v2, None, None, None, None) == api.ERR_SHAPE
+    p2 = bec_amd.params2d(gs_rule=2)
+    assert L.bec_run_2d(C.byref(p2), m2, v2, None, None, None, None) == api.ERR_INVALID
     # context-less calls
     assert L.bec_propagate(None, 1, 0.0, 0, None) == api.ERR_INVALID
     assert L.bec_observables(None, 0, None) == api.ERR_INVALID
