@@ -84,6 +84,9 @@ int setup_work(GsWork &w, const bec_calc_params *params, int batch, int prec, hi
     int st = group_setup(w.g, c0, prec, ntraj, false);
     if (st != BEC_OK) return st;
     DevGroup &d = w.g.dev;
+    // small FP64 batches: 16 pencils per z workgroup (twice the workgroups of the
+    // default 32, each with half the work); a solve of one set is latency-bound
+    if (prec == BEC_FP64 && (long)ntraj * c0.nvx * c0.nvy / 32 < 256) d.pw = 16;
     // one parameter record per set
     void *p = d.params;
     dev_free(p);

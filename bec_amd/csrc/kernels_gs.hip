@@ -260,25 +260,32 @@ template <typename T, int N> struct OpGsNorm : SkipDone {
 
 // ---------------------------------------------------------- scalar kernels
 
+// Sum of one trajectory's per-workgroup partials, by one wave: lane l adds
+// slots l, l + 64, ... and the lanes combine through shuffles (a fixed order,
+// so results are reproducible).  Every lane receives the total.  The scalar
+// kernels below run one wave per trajectory: their loads are issued in
+// parallel instead of one dependent chain per thread.
 __device__ __forceinline__ double psum(const double *partial, int per, int nsum, int traj, int k)
 {
     const double *p = partial + (long)traj * per * nsum + k;
     double s = 0.0;
-    for (int j = 0; j < per; ++j) s += p[(long)j * nsum];
-    return s;
+    for (int j = (int)threadIdx.x; j < per; j += 64) s += p[(long)j * nsum];
+    return group_sum<64>(s);
 }
 
 __global__ void scalar_ref_kernel(const double *partial, int per, int ntraj, const TrajParams *P, int rec_shift,
                                   GsScalars sc, double tol, int use_rule)
 {
-    const int traj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int traj = blockIdx.x;   // one wave per trajectory
     if (traj >= ntraj || sc.done[traj]) return;
+    const bool lead = threadIdx.x == 0;
     const TrajParams &p = P[traj >> rec_shift];
     const double dV = dv_of(p);
     const double D = psum(partial, per, 4, traj, 0), A = psum(partial, per, 4, traj, 1);
     const double B = psum(partial, per, 4, traj, 2), C = psum(partial, per, 4, traj, 3);
     const double s2 = p.N0 / (D * dV);
     const double E = dV * (s2 * (A + C) + p.g11 * s2 * s2 * B / 2.0);
+    if (!lead) return;
     sc.iters[traj] += 1;
     if (use_rule && fabs((E - sc.e_prev[traj]) / E) < tol) sc.done[traj] = 1;
     if (!(E == E) || !(s2 == s2)) sc.done[traj] = 2;   // non-finite
@@ -289,17 +296,20 @@ __global__ void scalar_ref_kernel(const double *partial, int per, int ntraj, con
 
 __global__ void scalar_mu_kernel(const double *partial, int per, int ntraj, GsScalars sc)
 {
-    const int traj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int traj = blockIdx.x;   // one wave per trajectory
     if (traj >= ntraj || sc.done[traj]) return;
-    sc.mu[traj] = psum(partial, per, 5, traj, 0) / psum(partial, per, 5, traj, 1);
+    const bool lead = threadIdx.x == 0;
+    const double mu = psum(partial, per, 5, traj, 0) / psum(partial, per, 5, traj, 1);
+    if (lead) sc.mu[traj] = mu;
 }
 
 // Pair mode: both components of a set take max(r_a, r_b), so they stop together.
 __global__ void scalar_residual_kernel(const double *ph, const double *pr, int per, int ntraj, GsScalars sc,
                                        double tol, int check, int pair)
 {
-    const int traj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int traj = blockIdx.x;   // one wave per trajectory
     if (traj >= ntraj || sc.done[traj]) return;
+    const bool lead = threadIdx.x == 0;
     const double nn = psum(ph, per, 5, traj, 1);
     const double rr = psum(pr, per, 1, traj, 0);
     double r = sqrt(rr / nn) / fabs(sc.mu[traj]);
@@ -308,6 +318,7 @@ __global__ void scalar_residual_kernel(const double *ph, const double *pr, int p
         const double rq = sqrt(psum(pr, per, 1, q, 0) / psum(ph, per, 5, q, 1)) / fabs(sc.mu[q]);
         r = (r == r && rq == rq) ? fmax(r, rq) : rq + r;   // NaN propagates
     }
+    if (!lead) return;
     sc.res[traj] = r;
     if (!(r == r)) sc.done[traj] = 2;
     else if (check && r < tol) sc.done[traj] = 1;
@@ -316,10 +327,12 @@ __global__ void scalar_residual_kernel(const double *ph, const double *pr, int p
 __global__ void scalar_norm_kernel(const double *partial, int per, int ntraj, const TrajParams *P, int rec_shift,
                                    GsScalars sc, int count_iter)
 {
-    const int traj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int traj = blockIdx.x;   // one wave per trajectory
     if (traj >= ntraj || sc.done[traj]) return;
+    const bool lead = threadIdx.x == 0;
     const TrajParams &p = P[traj >> rec_shift];
     const double n = psum(partial, per, 1, traj, 0) * dv_of(p);
+    if (!lead) return;
     sc.s[traj] = sqrt((sc.target ? sc.target[traj] : p.N0) / n);
     if (count_iter) sc.iters[traj] += 1;
     if (!(n == n)) sc.done[traj] = 2;
@@ -439,7 +452,7 @@ hipError_t gs_xy_measure_ref(const DevGroup &g, const void *psi, const void *kco
 hipError_t gs_scalar_ref(const DevGroup &g, const double *partial, const GsScalars &sc, double tol, int use_rule,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL(scalar_ref_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial, xy_wg_per_traj(g),
+    hipLaunchKernelGGL(scalar_ref_kernel, dim3(g.ntraj), dim3(64), 0, s, partial, xy_wg_per_traj(g),
                        g.ntraj, g.params, g.rec_shift, sc, tol, use_rule);
     return hipGetLastError();
 }
@@ -479,7 +492,7 @@ hipError_t gs_xy_hamiltonian(const DevGroup &g, const void *psi, void *w, const 
 
 hipError_t gs_scalar_mu(const DevGroup &g, const double *partial, const GsScalars &sc, hipStream_t s)
 {
-    hipLaunchKernelGGL(scalar_mu_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial, xy_wg_per_traj(g),
+    hipLaunchKernelGGL(scalar_mu_kernel, dim3(g.ntraj), dim3(64), 0, s, partial, xy_wg_per_traj(g),
                        g.ntraj, sc);
     return hipGetLastError();
 }
@@ -503,7 +516,7 @@ hipError_t gs_xy_residual(const DevGroup &g, const void *psi, void *w, const GsS
 hipError_t gs_scalar_residual(const DevGroup &g, const double *partial_h, const double *partial_r,
                               const GsScalars &sc, double tol, int check, hipStream_t s)
 {
-    hipLaunchKernelGGL(scalar_residual_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial_h, partial_r,
+    hipLaunchKernelGGL(scalar_residual_kernel, dim3(g.ntraj), dim3(64), 0, s, partial_h, partial_r,
                        xy_wg_per_traj(g), g.ntraj, sc, tol, check, sc.pair);
     return hipGetLastError();
 }
@@ -528,7 +541,7 @@ hipError_t gs_xy_update(const DevGroup &g, void *psi, void *prev, const void *w,
 
 hipError_t gs_scalar_norm(const DevGroup &g, const double *partial, const GsScalars &sc, hipStream_t s)
 {
-    hipLaunchKernelGGL(scalar_norm_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial, xy_wg_per_traj(g),
+    hipLaunchKernelGGL(scalar_norm_kernel, dim3(g.ntraj), dim3(64), 0, s, partial, xy_wg_per_traj(g),
                        g.ntraj, g.params, g.rec_shift, sc, 1);
     return hipGetLastError();
 }
@@ -544,7 +557,7 @@ hipError_t gs_norm_only(const DevGroup &g, const void *psi, const GsScalars &sc,
         return launch_xy<T, N>(g, op, g.ntraj, partial, s);
     });
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(scalar_norm_kernel, dim3((g.ntraj + 63) / 64), dim3(64), 0, s, partial, xy_wg_per_traj(g),
+    hipLaunchKernelGGL(scalar_norm_kernel, dim3(g.ntraj), dim3(64), 0, s, partial, xy_wg_per_traj(g),
                        g.ntraj, g.params, g.rec_shift, sc, 0);
     return hipGetLastError();
 }
