@@ -182,3 +182,32 @@ def test_uneven_groups_match_single_group_runs(bec, oracle, gs):
             a, b = s.get_state(bec.SPACE_K, group=grp)
             a1, b1 = single.get_state(bec.SPACE_K)
             assert rel_l2(a, a1) < 1e-6 and rel_l2(b, b1) < 1e-6
+
+
+@pytest.mark.parametrize("nxy,nz", [(16, 64), (16, 256), (32, 64), (32, 128), (32, 256)])
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
+def test_propagate_every_lattice_matches_oracle(bec, oracle, nxy, nz, prec, tol):
+    """Every supported lattice other than the headline one, with an odd trajectory
+    count (no XCD placement) and with its own ground state (split step, reference
+    rule, on the GPU in FP64 and checked to equal the oracle's first)."""
+    m = config1_model(oracle, N=30000 if nxy == 16 else 70000, nvx=nxy, nvy=nxy, nvz=nz, ne=3, detuning=-41.0)
+    c, oc = gpu_calc(bec, **m), oracle_calc(oracle, **m)
+    c1 = gpu_calc(bec, **dict(m, ne=1))
+    with bec.Solver(c1, precision=1) as g:
+        psi, rep, st = g.calculate_steady_state(method=bec.GS_GRADIENT, rule=bec.GS_RULE_RESIDUAL, tol=1e-6,
+                                                precision=1, max_iter=20000)
+    assert st == 0
+    assert oracle.residual(oc, psi)[0] < 1.05e-6
+    nsteps = 10
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(psi, SEED, equilibrate=True)
+        s.propagate(nsteps)
+        ax, bx = s.get_state(bec.SPACE_X)
+        obs = s.observables()
+    oa, ob = oracle.init_evolution(oc, psi, SEED, equilibrate=True)
+    oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, nsteps)
+    oax, obx = oracle.to_xspace(oc, oa), oracle.to_xspace(oc, ob)
+    ea = rel_l2(ax, oax, axis=(1, 2, 3))
+    eb = rel_l2(bx, obx, axis=(1, 2, 3))
+    assert max(ea.max(), eb.max()) < tol, (ea, eb)
+    assert abs(obs["visibility_grid"] - oracle.visibility_grid(oc, oax, obx)) < max(tol, 1e-10) * 10
