@@ -201,16 +201,23 @@ int split_step_iteration(GsWork &w, const bec_gs_options &o, int it, hipStream_t
     return BEC_OK;
 }
 
+// One gradient-flow iteration in six launches: the per-trajectory scalars
+// (mu, the residual test, the renormalisation of the previous update) are
+// evaluated inside the passes that consume them.  Iteration 0 starts from
+// the normalised guess; later ones renormalise the previous update first.
 int gradient_iteration(GsWork &w, const bec_gs_options &o, int it, double alpha, hipStream_t s)
 {
     const DevGroup &d = w.g.dev;
     GsTables t{w.exy, w.ez};
     const int check = o.rule == BEC_GS_RULE_RESIDUAL && (it % std::max(1, o.check_every) == 0);
-    GS_HIP(residual_pass(w, w.sc, o.tol, check, s));
+    GS_HIP(gs_xy_forward(d, w.X, w.W, w.sc, s, it > 0 ? w.pn : nullptr));
+    GS_HIP(gs_z_diag(d, w.W, w.sc, t, 0, 0.0, nullptr, s));
+    GS_HIP(gs_xy_hamiltonian(d, w.X, w.W, w.sc, w.ph, s));
+    GS_HIP(gs_xy_residual(d, w.X, w.W, w.sc, w.pr, s, w.ph));
     // preconditioner shift: the option, else each record's Thomas-Fermi mu
-    GS_HIP(gs_z_diag(d, w.W, w.sc, t, 1, alpha, alpha > 0 ? nullptr : w.mu_tf, s));
+    const GsResidualTest test{w.ph, w.pr, o.tol, check};
+    GS_HIP(gs_z_diag(d, w.W, w.sc, t, 1, alpha, alpha > 0 ? nullptr : w.mu_tf, s, &test));
     GS_HIP(gs_xy_update(d, w.X, w.P, w.W, w.sc, o.tau, o.beta, w.pn, s));
-    GS_HIP(gs_scalar_norm(d, w.pn, w.sc, s));
     return BEC_OK;
 }
 
@@ -365,6 +372,8 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
     (void)hipEventDestroy(e1);
     if (st != BEC_OK) return st;
     if (o.method == BEC_GS_SPLIT_STEP) GS_HIP(run_xy_fft(d, w.P, w.X, batch, -1, 1.0, s));
+    // the last update's renormalisation (trajectories still running; done ones skip)
+    if (o.method == BEC_GS_GRADIENT && it > 0) GS_HIP(gs_scalar_norm(d, w.pn, w.sc, s));
     GS_HIP(gs_apply_scale(d, w.X, w.sc, s));
     std::vector<int> iters(ntraj);
     GS_HIP(hipMemcpyAsync(iters.data(), w.sc.iters, sizeof(int) * ntraj, hipMemcpyDeviceToHost, s));

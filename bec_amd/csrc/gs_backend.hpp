@@ -37,19 +37,33 @@ hipError_t gs_xy_measure_ref(const DevGroup &g, const void *psi, const void *kco
 hipError_t gs_scalar_ref(const DevGroup &g, const double *partial, const GsScalars &sc, double tol,
                          int use_rule, hipStream_t s);
 
+// The residual test of scalar_residual_kernel, evaluated inside the next pass
+// (every workgroup from the partials; the first records res and done).
+struct GsResidualTest {
+    const double *ph;   // Hamiltonian-pass partials
+    const double *pr;   // residual-pass partials
+    double tol;
+    int check;
+};
+
 // ---- gradient flow / residual machinery (x-space iterate psi)
-hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s);
+// pn != nullptr: first renormalise with the scale of the previous update's
+// partials (the work of gs_scalar_norm, counting the iteration)
+hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s,
+                         const double *pn = nullptr);
 // variant 0: multiply by k^2/2 / nz; 1: by 1 / ((alpha + k^2/2) nz), alpha
-// taken per record from alpha_rec when it is not nullptr
+// taken per record from alpha_rec when it is not nullptr; test != nullptr:
+// run the residual test first and skip the trajectories it stops
 hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTables &t, int variant,
-                     double alpha, const double *alpha_rec, hipStream_t s);
+                     double alpha, const double *alpha_rec, hipStream_t s, const GsResidualTest *test = nullptr);
 // h = K psi + (V + g11 |psi|^2) psi; partials {Re<psi,h>, <psi,psi>, <psi,K psi>, sum V n, sum n^2}
 hipError_t gs_xy_hamiltonian(const DevGroup &g, const void *psi, void *w, const GsScalars &sc,
                              double *partial, hipStream_t s);
 hipError_t gs_scalar_mu(const DevGroup &g, const double *partial, const GsScalars &sc, hipStream_t s);
-// r = h - mu psi; partial {|r|^2}; w <- F_xy(r)
+// r = h - mu psi; partial {|r|^2}; w <- F_xy(r); ph != nullptr: mu from the
+// Hamiltonian-pass partials first (the work of gs_scalar_mu, recorded in sc.mu)
 hipError_t gs_xy_residual(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, double *partial,
-                          hipStream_t s);
+                          hipStream_t s, const double *ph = nullptr);
 hipError_t gs_scalar_residual(const DevGroup &g, const double *partial_h, const double *partial_r,
                               const GsScalars &sc, double tol, int check, hipStream_t s);
 // psi_new = s psi - tau d + beta (s psi - prev); prev <- s psi; partial {|psi_new|^2}

@@ -16,6 +16,18 @@ namespace {
 
 __device__ __forceinline__ double dv_of(const TrajParams &p) { return p.dx * p.dy * p.dz; }
 
+// Sum of one trajectory's per-workgroup partials, by one wave: lane l adds
+// slots l, l + 64, ... and the lanes combine through shuffles (a fixed order,
+// so results are reproducible and every wave that computes it gets the same
+// bits).  Every lane receives the total.
+__device__ __forceinline__ double psum(const double *partial, int per, int nsum, int traj, int k)
+{
+    const double *p = partial + (long)traj * per * nsum + k;
+    double s = 0.0;
+    for (int j = (int)(threadIdx.x & 63); j < per; j += 64) s += p[(long)j * nsum];
+    return group_sum<64>(s);
+}
+
 // --------------------------------------------------------------- z-passes
 
 template <typename T, int NZ, int R1> struct OpGsZImag : SkipDone {
@@ -46,6 +58,35 @@ template <typename T, int NZ, int R1> struct OpGsZImag : SkipDone {
     }
 };
 
+// Fused residual test (ph != nullptr): every workgroup evaluates its
+// trajectory's residual from the partials of the Hamiltonian and residual
+// passes (the same fixed-order sums as scalar_residual_kernel), stops on
+// convergence or a non-finite value, and the first workgroup records it.
+struct FusedResidual {
+    const double *ph = nullptr, *pr = nullptr;   // partials {.., <psi,psi>, ..}, {|r|^2}
+    int per = 0, check = 0, pair = 0;
+    double tol = 0.0;
+    GsScalars sc{};
+    bool stop = false;
+    __device__ __forceinline__ void run(int traj, bool first, const int *done)
+    {
+        if (!ph || (done && done[traj])) return;
+        const double mu = sc.mu[traj];
+        double r = sqrt(psum(pr, per, 1, traj, 0) / psum(ph, per, 5, traj, 1)) / fabs(mu);
+        if (pair) {
+            const int q = traj ^ 1;
+            const double rq = sqrt(psum(pr, per, 1, q, 0) / psum(ph, per, 5, q, 1)) / fabs(sc.mu[q]);
+            r = (r == r && rq == rq) ? fmax(r, rq) : rq + r;
+        }
+        const int flag = !(r == r) ? 2 : (check && r < tol) ? 1 : 0;
+        stop = flag != 0;
+        if (first && threadIdx.x == 0) {
+            sc.res[traj] = r;
+            if (flag) sc.done[traj] = flag;
+        }
+    }
+};
+
 template <typename T, int NZ, int R1> struct OpGsZDiag : SkipDone {
     static constexpr int NC = 1;
     static constexpr int NW = NZ / R1, KPT = R1 / NW;
@@ -54,6 +95,9 @@ template <typename T, int NZ, int R1> struct OpGsZDiag : SkipDone {
     const double *alpha_rec;   // per-record preconditioner shift (nullptr: alpha)
     int nxy, rec_shift, variant;
     double alpha;
+    FusedResidual fr;
+    __device__ __forceinline__ void begin(int traj, bool first) { fr.run(traj, first, done); }
+    __device__ __forceinline__ bool skip(int traj) const { return (done && done[traj]) || fr.stop; }
     __device__ const cpx<T> *src(int) const { return w; }
     __device__ cpx<T> *dst(int) const { return w; }
     __device__ __forceinline__ void kspace(int, cpx<T> (&X)[KPT][NW], int kk0, int q, int traj) const
@@ -136,17 +180,44 @@ template <typename T, int N> struct OpGsMeasureRef : SkipDone {
 };
 
 // x-space psi * s / (NX NY) -> forward xy transform (x -> hybrid)
+// Fused renormalisation (pn != nullptr): the scale of the previous update,
+// s = sqrt(N_target / (dV sum |psi|^2)), from its partials (the sums of
+// scalar_norm_kernel); the first workgroup records s and counts the iteration.
 template <typename T, int N> struct OpGsForward : SkipDone {
     static constexpr int NC = 1, SIGN_IN = 0, SIGN_OUT = +1, NSUM = 0;
     static constexpr bool WRITE = true;
     const cpx<T> *psi;
     cpx<T> *w;
     const double *scale;
+    const double *pn = nullptr;
+    int per = 0, rec_shift = 0;
+    const TrajParams *P = nullptr;
+    GsScalars sc{};
+    double s_v = 1.0;
+    bool bad = false;
+    __device__ __forceinline__ void begin(int traj, bool first)
+    {
+        if (done && done[traj]) return;
+        if (!pn) {
+            s_v = scale[traj];
+            return;
+        }
+        const TrajParams &p = P[traj >> rec_shift];
+        const double n = psum(pn, per, 1, traj, 0) * dv_of(p);
+        s_v = sqrt((sc.target ? sc.target[traj] : p.N0) / n);
+        bad = !(n == n);
+        if (first && threadIdx.x == 0) {
+            sc.s[traj] = s_v;
+            sc.iters[traj] += 1;
+            if (bad) sc.done[traj] = 2;
+        }
+    }
+    __device__ __forceinline__ bool skip(int traj) const { return (done && done[traj]) || bad; }
     __device__ const cpx<T> *src(int) const { return psi; }
     __device__ cpx<T> *dst(int) const { return w; }
     __device__ __forceinline__ void apply(cpx<T> (&r)[1][N], int, int, int traj, long, double *) const
     {
-        const T s = (T)(scale[traj] / (N * N));
+        const T s = (T)(s_v / (N * N));
 #pragma unroll
         for (int x = 0; x < N; ++x) r[0][x] = cscale<T>(r[0][x], s);
     }
@@ -195,17 +266,34 @@ template <typename T, int N> struct OpGsHam : SkipDone {
 };
 
 // r = h - mu psi_s ; partial |r|^2 ; r / (NX NY) -> forward xy transform
+// Fused Rayleigh quotient (ph != nullptr): mu = Re<psi,h>/<psi,psi> from the
+// Hamiltonian pass's partials (the sums of scalar_mu_kernel); the first
+// workgroup records it.
 template <typename T, int N> struct OpGsResid : SkipDone {
     static constexpr int NC = 1, SIGN_IN = 0, SIGN_OUT = +1, NSUM = 1;
     static constexpr bool WRITE = true;
     const cpx<T> *psi;
     cpx<T> *w;
     const double *scale, *mu;
+    const double *ph = nullptr;
+    int per = 0;
+    double *mu_out = nullptr;
+    double mu_v = 0.0;
+    __device__ __forceinline__ void begin(int traj, bool first)
+    {
+        if (done && done[traj]) return;
+        if (!ph) {
+            mu_v = mu[traj];
+            return;
+        }
+        mu_v = psum(ph, per, 5, traj, 0) / psum(ph, per, 5, traj, 1);
+        if (first && threadIdx.x == 0) mu_out[traj] = mu_v;
+    }
     __device__ const cpx<T> *src(int) const { return w; }
     __device__ cpx<T> *dst(int) const { return w; }
     __device__ __forceinline__ void apply(cpx<T> (&r)[1][N], int, int, int traj, long row, double *s) const
     {
-        const double sc = scale[traj], m = mu[traj];
+        const double sc = scale[traj], m = mu_v;
         const T inv = (T)(1.0 / (N * N));
         static_for<0, N>([&](auto X_) {
             constexpr int x = decltype(X_)::value;
@@ -260,18 +348,8 @@ template <typename T, int N> struct OpGsNorm : SkipDone {
 
 // ---------------------------------------------------------- scalar kernels
 
-// Sum of one trajectory's per-workgroup partials, by one wave: lane l adds
-// slots l, l + 64, ... and the lanes combine through shuffles (a fixed order,
-// so results are reproducible).  Every lane receives the total.  The scalar
-// kernels below run one wave per trajectory: their loads are issued in
-// parallel instead of one dependent chain per thread.
-__device__ __forceinline__ double psum(const double *partial, int per, int nsum, int traj, int k)
-{
-    const double *p = partial + (long)traj * per * nsum + k;
-    double s = 0.0;
-    for (int j = (int)threadIdx.x; j < per; j += 64) s += p[(long)j * nsum];
-    return group_sum<64>(s);
-}
+// The scalar kernels below run one wave per trajectory (psum): their loads
+// are issued in parallel instead of one dependent chain per thread.
 
 __global__ void scalar_ref_kernel(const double *partial, int per, int ntraj, const TrajParams *P, int rec_shift,
                                   GsScalars sc, double tol, int use_rule)
@@ -395,7 +473,7 @@ hipError_t gs_z_imag(const DevGroup &g, void *psi, const GsScalars &sc, void *kc
 }
 
 hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTables &t, int variant, double alpha,
-                     const double *alpha_rec, hipStream_t s)
+                     const double *alpha_rec, hipStream_t s, const GsResidualTest *test)
 {
     int pw = g.pw ? g.pw : default_pw(g.prec, g.nx, g.nz);
     return dispatch(g, [&](auto Tt, auto, auto NZt) -> hipError_t {
@@ -412,6 +490,15 @@ hipError_t gs_z_diag(const DevGroup &g, void *w, const GsScalars &sc, const GsTa
         op.variant = variant;
         op.alpha = alpha;
         op.alpha_rec = alpha_rec;
+        if (test) {
+            op.fr.ph = test->ph;
+            op.fr.pr = test->pr;
+            op.fr.per = xy_wg_per_traj(g);
+            op.fr.check = test->check;
+            op.fr.pair = sc.pair;
+            op.fr.tol = test->tol;
+            op.fr.sc = sc;
+        }
         return with_pw(pw, [&](auto PWt) { return launch_z<T, NZ, decltype(PWt)::value, +1, true>(g, op, g.ntraj, s); });
     });
 }
@@ -457,7 +544,8 @@ hipError_t gs_scalar_ref(const DevGroup &g, const double *partial, const GsScala
     return hipGetLastError();
 }
 
-hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s)
+hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, hipStream_t s,
+                         const double *pn)
 {
     return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
         using T = typename decltype(Tt)::type;
@@ -467,6 +555,11 @@ hipError_t gs_xy_forward(const DevGroup &g, const void *psi, void *w, const GsSc
         op.psi = (const cpx<T> *)psi;
         op.w = (cpx<T> *)w;
         op.scale = sc.s;
+        op.pn = pn;
+        op.per = xy_wg_per_traj(g);
+        op.P = g.params;
+        op.rec_shift = g.rec_shift;
+        op.sc = sc;
         return launch_xy<T, N>(g, op, g.ntraj, nullptr, s);
     });
 }
@@ -498,7 +591,7 @@ hipError_t gs_scalar_mu(const DevGroup &g, const double *partial, const GsScalar
 }
 
 hipError_t gs_xy_residual(const DevGroup &g, const void *psi, void *w, const GsScalars &sc, double *partial,
-                          hipStream_t s)
+                          hipStream_t s, const double *ph)
 {
     return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {
         using T = typename decltype(Tt)::type;
@@ -509,6 +602,9 @@ hipError_t gs_xy_residual(const DevGroup &g, const void *psi, void *w, const GsS
         op.w = (cpx<T> *)w;
         op.scale = sc.s;
         op.mu = sc.mu;
+        op.ph = ph;
+        op.per = xy_wg_per_traj(g);
+        op.mu_out = sc.mu;
         return launch_xy<T, N>(g, op, g.ntraj, partial, s);
     });
 }

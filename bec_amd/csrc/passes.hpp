@@ -122,6 +122,7 @@ xy_kernel(Op op, XyGeom g, double *partial)
     const long plane = blk * PPW + pl;
     const int z = (int)(plane % g.nz);
     const int traj = (int)(plane / g.nz);
+    op.begin(traj, blk % (g.nz / PPW) == 0);
     if (op.skip(traj)) return;   // uniform: a workgroup never spans two trajectories
     cpx<T> *tile = lds + pl * N * (N + 1);
     const long base = plane * (long)(N * N);
@@ -171,13 +172,17 @@ xy_kernel(Op op, XyGeom g, double *partial)
 
 // ------------------------------------------------------------ xy operators
 
-// Operators that never skip a trajectory.
+// Operators that never skip a trajectory.  begin(traj, first) runs once per
+// workgroup, by every thread, before skip(): operators may compute per-
+// trajectory scalars there (first: the workgroup is its trajectory's first).
 struct NoSkip {
+    __device__ __forceinline__ void begin(int, bool) {}
     __device__ __forceinline__ bool skip(int) const { return false; }
 };
 // Operators that skip trajectories whose done flag is set.
 struct SkipDone {
     const int *done;
+    __device__ __forceinline__ void begin(int, bool) {}
     __device__ __forceinline__ bool skip(int traj) const { return done && done[traj]; }
 };
 
@@ -402,6 +407,7 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
     const int cb = blockIdx.y;
     const long blk = g.xcd ? xcd_remap(blockIdx.x, g.tiles_per_traj) : (long)blockIdx.x;
     const int traj = (int)(blk / g.tiles_per_traj);
+    op.begin(traj, blk % g.tiles_per_traj == 0 && cb == 0);
     if (op.skip(traj)) return;
     const int q = (int)(blk % g.tiles_per_traj) * PW + p;
     const long base = (long)traj * NZ * g.nxy + q;
