@@ -275,6 +275,14 @@ hipError_t G(const bec_fno *f, const void *A, int abf, const void *B, int bbf, f
     const long ea = max_off(ab, batch) + max_off(am, M) + max_off(ak, K);
     const long eb = max_off(bb, batch) + max_off(bk, K) + max_off(bn, N);
     const long ec = max_off(cb, batch) + max_off(cm, M) + max_off(cn, N);
+    // A GEMM that fills few workgroups (small batches: the transforms of a
+    // single-sample forward pass) walks its K tiles in one dependent chain
+    // per workgroup; split K so the chip fills, reduced in a fixed order.
+    const long wgs = ((long)(N + 63) / 64) * ((M + 63) / 64) * batch;
+    if (ksplit == 1 && wgs < 256 && K >= 128) {
+        ksplit = (int)std::min<long>(std::min<long>(8, (512 + wgs - 1) / wgs), K / 64);
+        work = f->work;
+    }
     bool ok = ea < room(f, A, abf ? 2 : 4) && eb < room(f, B, bbf ? 2 : 4) && ec < room(f, C, 4);
     if (bias) ok = ok && N <= room(f, bias, 4);
     if (ksplit > 1) ok = ok && (long)ksplit * batch * M * N <= room(f, work, 4);
@@ -583,7 +591,9 @@ int create(const bec_fno_config &c, uint64_t seed, bec_fno *f)
     FA(f->dB, fb * rows * c.hidden);
     FA(f->dE, fb * f->modes * 4 * W * W);
     long mn = std::max(std::max((long)c.c_in * W, W * W), std::max(W * (long)c.hidden, (long)c.hidden * c.c_out));
-    FA(f->work, fb * KSPLIT * mn);
+    // split-K partials: the weight gradients (KSPLIT layers of at most mn) and the
+    // small-batch splits of G (at most 8 layers of < 256 64 x 64 tiles)
+    FA(f->work, fb * std::max((long)KSPLIT * mn, 8L * 256 * 64 * 64));
     FA(f->loss_part, sizeof(double) * ((rows * c.c_out + 255) / 256));
 #undef FA
     std::vector<float> p;
