@@ -96,6 +96,23 @@ class Report2D(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Params1D(C.Structure):
+    """bec_1d_params: the file-scope constants of programs/bec_wigner_2d.cpp/wigner_1d.cpp:93-130."""
+    _fields_ = [("zmax", C.c_double), ("dz", C.c_double),
+                ("g11", C.c_double), ("g12", C.c_double), ("g22", C.c_double),
+                ("E", C.c_double), ("mu", C.c_double), ("Va", C.c_double), ("Vb", C.c_double),
+                ("dt", C.c_double), ("ntG", C.c_int), ("itmax", C.c_int), ("ntW", C.c_int), ("ne", C.c_int),
+                ("n0", C.c_double), ("seed", C.c_uint64), ("noise_dev", C.c_double)]
+
+
+class Report1D(C.Structure):
+    _fields_ = [("N0", C.c_double), ("nv", C.c_int), ("gs_residual", C.c_double), ("gs_mu", C.c_double),
+                ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Obs(C.Structure):
     _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
                 ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
@@ -133,6 +150,8 @@ EXPORTS = {
                                                   _P, _P, C.POINTER(Gs2Report)]),
     "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
     "bec_run_2d": (C.c_int, [C.POINTER(Params2D), _P, _P, _P, _P, _P, C.POINTER(Report2D)]),
+    "bec_1d_cells": (C.c_int, [C.POINTER(Params1D)]),
+    "bec_run_1d": (C.c_int, [C.POINTER(Params1D), _P, _P, _P, _P, _P, _P, C.POINTER(Report1D)]),
     "bec_init_evolution": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int]),
     "bec_propagate": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _P]),
     "bec_propagate_graph": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
@@ -296,6 +315,46 @@ def run_2d(p, want_avn=True):
     check(lib().bec_run_2d(C.byref(p), _ptr(m), _ptr(var), _ptr(vis), _ptr(gs),
                            _ptr(avn) if want_avn else None, C.byref(rep)), "bec_run_2d")
     return dict(m=m.reshape(2, nm), var=var.reshape(2, nm), vis=vis, gs=gs, avn=avn, report=rep.as_dict())
+
+
+def params1d(**kw):
+    """The 1D program's constants (programs/bec_wigner_2d.cpp/wigner_1d.cpp:93-130); keyword arguments
+    override.  tmaxGP / tmaxWig give ntG = 1 + (int)(tmaxGP/dt) and ntW = 1 + (int)(tmaxWig/dt) as the
+    program derives them."""
+    d = dict(zmax=20.0, dz=1.0, g11=6.18, g12=6.03, g22=5.88, E=2.5, mu=26.8, Va=0.5, Vb=1.0, dt=0.01, ntG=None,
+             itmax=3, ntW=None, ne=20, n0=1.0, seed=0x5EED, noise_dev=None, tmaxGP=1.0, tmaxWig=30.0)
+    d.update(kw)
+    tG, tW = d.pop("tmaxGP"), d.pop("tmaxWig")
+    if d["ntG"] is None:
+        d["ntG"] = 1 + int(tG / d["dt"])
+    if d["ntW"] is None:
+        d["ntW"] = 1 + int(tW / d["dt"])
+    if d["noise_dev"] is None:
+        d["noise_dev"] = np.sqrt(1.0 / d["dz"]) / 2.0
+    return Params1D(**d)
+
+
+def cells_1d(p):
+    """Lattice cells of the 1D program's parameters (bec_1d_cells)."""
+    n = lib().bec_1d_cells(C.byref(p))
+    check(-n if n < 0 else OK, "bec_1d_cells")
+    return n
+
+
+def run_1d(p):
+    """The 1D program on the GPU: ground state, then ne trajectories (bec_run_1d)."""
+    nv, nt = cells_1d(p), p.ntW
+    m = np.zeros(2 * nt)
+    var = np.zeros(2 * nt)
+    vis = np.zeros(nt)
+    gs = np.empty(nv, dtype=np.complex128)
+    avn = np.empty(nv)
+    ntot = np.empty(p.ntG)
+    rep = Report1D()
+    check(lib().bec_run_1d(C.byref(p), _ptr(m), _ptr(var), _ptr(vis), _ptr(gs), _ptr(avn), _ptr(ntot),
+                           C.byref(rep)), "bec_run_1d")
+    return dict(m=m.reshape(2, nt), var=var.reshape(2, nt), vis=vis, gs=gs, avn=avn, ntot=ntot,
+                report=rep.as_dict())
 
 
 class Solver:

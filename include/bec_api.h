@@ -218,6 +218,51 @@ typedef struct bec_2d_report {
 int bec_run_2d(const bec_2d_params *p, double *m, double *var, double *vis, double *gs_x, double *avn,
                bec_2d_report *rep);
 
+/* ------------------------------------------------------- the 1D program
+ * programs/bec_wigner_2d.cpp/wigner_1d.cpp on the GPU, FP64 (the scheme of the
+ * Scilab prototype programs/bec_wigner_1d.scilab/WignerSplitFFT.sce):
+ * calculate_steady_state (:166-243) then main_loop (:245-445) for ne
+ * trajectories, one wavefront per trajectory.  Field meanings are the
+ * program's file-scope constants (:93-130): the lattice has
+ * nv = 1 + 2 (int)(zmax / dz) cells at z = -zmax + dz j (any nv from 1 to
+ * 512, the program's is 41), ntG counts the ground-state loop including its
+ * initial pass (ntG - 1 steps), and the ensemble loop takes ntW - 1
+ * equilibration steps, the pulse, then ntW steps each followed by a sample.
+ * The noise is Philox (counter {cell, trajectory, 0, 0}, key seed) with
+ * deviation noise_dev per quadrature (the program: sqrt(1/dz)/2). */
+typedef struct bec_1d_params {
+    double zmax, dz;
+    double g11, g12, g22, E, mu;
+    double Va, Vb;
+    double dt;
+    int ntG, itmax;
+    int ntW;
+    int ne;
+    double n0;        /* uniform start of the ground state */
+    uint64_t seed;
+    double noise_dev;
+} bec_1d_params;
+
+typedef struct bec_1d_report {
+    double N0;            /* atoms of the ground state: the last entry of its N series */
+    int nv;               /* lattice cells */
+    double gs_residual;   /* ||H psi - mu psi|| / (|mu| ||psi||) of the ground state */
+    double gs_mu;
+    double elapsed_ms;    /* device time of ground state + ensemble + reduction */
+} bec_1d_report;
+
+/* Lattice cells nv of these parameters, or minus a status code when they are invalid. */
+int bec_1d_cells(const bec_1d_params *p);
+
+/* m, var: [2][ntW] ensemble means of N1, N2 and their standard errors computed
+ * as wigner_1d.cpp:405-410 writes them (sqrt(var / (ne - 1)) with no absolute
+ * value: NaN for ne = 1); vis: [ntW] mean visibility 2|<a|b>| / (Na + Nb) before
+ * the second pulse (may be NULL); gs_x: [nv] complex ground state (may be NULL);
+ * avn: [nv] mean x-space density minus the vacuum term after the last step
+ * (may be NULL); ntot: [ntG] the ground-state loop's N series (may be NULL). */
+int bec_run_1d(const bec_1d_params *p, double *m, double *var, double *vis, double *gs_x, double *avn,
+               double *ntot, bec_1d_report *rep);
+
 /* ---------------------------------------------------------- evolution */
 /* initEvolution (wigner.cu:202-227): ground state + Philox vacuum noise of
  * variance 1/(2 dV) per mode (initializeEnsembles, wigner_kernel.cu:295-317),

@@ -986,3 +986,224 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
     tables2d_free(&t);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* 1D CPU program (bec_wigner_2d.cpp/wigner_1d.cpp)                     */
+
+/* The transform FFTW's plans compute (fftw_plan_dft_1d, any length), by its
+   definition: exponent index (j k) mod n, accumulated over j in order. */
+void orc_dft1d(orc_c *d, int n, int sign)
+{
+    orc_c *tw = malloc(sizeof(orc_c) * n), *out = malloc(sizeof(orc_c) * n);
+    for (int m = 0; m < n; m++) {
+        double ang = 2.0 * M_PI * (double)m / (double)n;
+        tw[m] = cmk(cos(ang), sign * sin(ang));
+    }
+    for (int k = 0; k < n; k++) {
+        orc_c acc = cmk(0.0, 0.0);
+        long idx = 0;
+        for (int j = 0; j < n; j++) {
+            acc = cadd(acc, cmul(d[j], tw[idx]));
+            idx += k;
+            if (idx >= n) idx -= n;
+        }
+        out[k] = acc;
+    }
+    memcpy(d, out, sizeof(orc_c) * n);
+    free(tw);
+    free(out);
+}
+
+/* nv = 1 + 2 (int)(zmax / dz) (wigner_1d.cpp:118) */
+int orc1d_nv(const orc1d_params *p) { return 1 + 2 * (int)(p->zmax / p->dz); }
+
+typedef struct {
+    int n;
+    double *z, *k2, *pG, *pa, *pb, *propG;
+    orc_c *propW;
+} tables1d;
+
+/* fill_propagation_matrices (wigner_1d.cpp:136-164) */
+static void tables1d_init(const orc1d_params *p, tables1d *t)
+{
+    int n = orc1d_nv(p);
+    double dk = M_PI / p->zmax;
+    t->n = n;
+    t->z = malloc(sizeof(double) * n); t->k2 = malloc(sizeof(double) * n);
+    t->pG = malloc(sizeof(double) * n); t->pa = malloc(sizeof(double) * n);
+    t->pb = malloc(sizeof(double) * n); t->propG = malloc(sizeof(double) * n);
+    t->propW = malloc(sizeof(orc_c) * n);
+    for (int j = 0; j < n; j++) {
+        double z = -p->zmax + p->dz * j;
+        double k = (2 * j > n) ? dk * (j - n) : dk * j;
+        t->z[j] = z;
+        t->k2[j] = k * k;
+        t->pG[j] = p->mu - z * z / 4.0;
+        t->pa[j] = p->mu - z * z / 4.0 + 0.5 * p->E;
+        t->pb[j] = p->mu - z * z / 4.0 - 0.5 * p->E;
+        t->propG[j] = exp(-k * k * p->dt / 2.0);
+        /* exp(prop (-i dt/2)) with prop = -k^2 */
+        t->propW[j] = cmk(cos(k * k * p->dt / 2.0), sin(k * k * p->dt / 2.0));
+    }
+}
+
+static void tables1d_free(tables1d *t)
+{
+    free(t->z); free(t->k2); free(t->pG); free(t->pa); free(t->pb); free(t->propG); free(t->propW);
+}
+
+/* x -> k: FFTW_BACKWARD (+i) and 1/n */
+static void to_k1d(orc_c *d, int n)
+{
+    orc_dft1d(d, n, +1);
+    for (int j = 0; j < n; j++) d[j] = cscale(d[j], 1.0 / n);
+}
+
+/* residual of (-d^2/dz^2 + z^2/4 + g11|psi|^2) psi = mu psi, relative to mu */
+double orc1d_residual(const orc1d_params *p, const orc_c *psi, double *mu_out)
+{
+    tables1d t;
+    tables1d_init(p, &t);
+    int n = t.n;
+    orc_c *h = malloc(sizeof(orc_c) * n);
+    memcpy(h, psi, sizeof(orc_c) * n);
+    to_k1d(h, n);
+    for (int j = 0; j < n; j++) h[j] = cscale(h[j], t.k2[j]);
+    orc_dft1d(h, n, -1);
+    double num = 0, den = 0;
+    for (int j = 0; j < n; j++) {
+        double v = (p->mu - t.pG[j]) + p->g11 * cnorm(psi[j]);
+        h[j] = cadd(h[j], cscale(psi[j], v));
+        num += psi[j].re * h[j].re + psi[j].im * h[j].im;
+        den += cnorm(psi[j]);
+    }
+    double mu = num / den, r = 0;
+    for (int j = 0; j < n; j++) r += cnorm(csub(h[j], cscale(psi[j], mu)));
+    free(h);
+    tables1d_free(&t);
+    if (mu_out) *mu_out = mu;
+    return sqrt(r / den) / fabs(mu);
+}
+
+/* calculate_steady_state (wigner_1d.cpp:166-243) + main_loop (wigner_1d.cpp:245-445).
+   The program's rand() Box-Muller deviates are replaced by the Philox stream of
+   orc_noise_cell (trajectory j, cell i): (z0, z1) for a, (z2, z3) for b. */
+int orc1d_run(const orc1d_params *p, double *m, double *var, double *vis, orc_c *gs_out, double *avn,
+              double *ntot, orc1d_report *rep)
+{
+    tables1d t;
+    tables1d_init(p, &t);
+    int n = t.n, ntW = p->ntW;
+    orc_c *a = malloc(sizeof(orc_c) * n), *b = malloc(sizeof(orc_c) * n);
+    orc_c *a1 = malloc(sizeof(orc_c) * n), *a0 = malloc(sizeof(orc_c) * n);
+    double *da = malloc(sizeof(double) * n);
+
+    /* ground state: uniform n0 in x-space, to k-space (:183-187) */
+    for (int j = 0; j < n; j++) a[j] = cmk(p->n0, 0.0);
+    to_k1d(a, n);
+    double N0 = 0;
+    for (int it = 0; it < p->ntG; it++) {
+        if (it > 0) {
+            for (int j = 0; j < n; j++) a[j] = cscale(a[j], t.propG[j]);
+            orc_dft1d(a, n, -1);
+            memcpy(a1, a, sizeof(orc_c) * n);
+            for (int j = 0; j < n; j++) da[j] = 0.0;
+            for (int iter = 0; iter < p->itmax; iter++)
+                for (int j = 0; j < n; j++) {
+                    da[j] = exp(p->dt / 2.0 * (t.pG[j] - p->g11 * cnorm(a[j])));
+                    a[j] = cscale(a1[j], da[j]);
+                }
+            for (int j = 0; j < n; j++) a[j] = cscale(a[j], da[j]);
+            to_k1d(a, n);
+            for (int j = 0; j < n; j++) a[j] = cscale(a[j], t.propG[j]);
+        }
+        double s = 0;
+        for (int j = 0; j < n; j++) s += cnorm(a[j]);
+        s *= p->dz * n;
+        if (ntot) ntot[it] = s;
+        N0 = s;
+    }
+    orc_dft1d(a, n, -1);
+    memcpy(a0, a, sizeof(orc_c) * n);
+    if (gs_out) memcpy(gs_out, a0, sizeof(orc_c) * n);
+    if (rep) {
+        rep->N0 = N0;
+        rep->nv = n;
+        rep->gs_residual = orc1d_residual(p, a0, &rep->gs_mu);
+    }
+
+    double V = (p->Va + p->Vb) / 2.0, r2 = sqrt(2.0);
+    for (int i = 0; i < 2 * ntW; i++) { m[i] = 0; var[i] = 0; }
+    if (vis) for (int i = 0; i < ntW; i++) vis[i] = 0;
+    if (avn) for (int j = 0; j < n; j++) avn[j] = 0;
+    double nd = p->noise_dev / 0.5;
+    for (int e = 0; e < p->ne; e++) {
+        for (int j = 0; j < n; j++) {   /* :281-297 */
+            double z[4];
+            orc_noise_cell(p->seed, (uint32_t)e, (uint32_t)j, 0u, z);
+            a[j] = cmk(a0[j].re + p->Va * nd * z[0], a0[j].im + p->Va * nd * z[1]);
+            b[j] = cmk(p->Vb * nd * z[2], p->Vb * nd * z[3]);
+        }
+        to_k1d(a, n);
+        to_k1d(b, n);
+        for (int nn = -ntW; nn < ntW; nn++) {
+            if (nn > -ntW) {
+                if (nn == 0) {   /* pi/2 pulse (:308-318) */
+                    for (int j = 0; j < n; j++) {
+                        orc_c bo = b[j];
+                        b[j] = cscale(csub(b[j], cmul(cmk(0, 1), a[j])), 1.0 / r2);
+                        a[j] = cscale(csub(a[j], cmul(cmk(0, 1), bo)), 1.0 / r2);
+                    }
+                }
+                for (int j = 0; j < n; j++) { a[j] = cmul(a[j], t.propW[j]); b[j] = cmul(b[j], t.propW[j]); }
+                orc_dft1d(a, n, -1);
+                orc_dft1d(b, n, -1);
+                for (int j = 0; j < n; j++) {   /* single-shot nonlinear step (:331-337) */
+                    double na = cnorm(a[j]), nb = cnorm(b[j]);
+                    double dA = -p->g11 * na - p->g12 * nb + t.pa[j];
+                    double dB = -p->g22 * nb - p->g12 * na + t.pb[j];
+                    a[j] = cmul(a[j], cexp_(cmk(0, -p->dt * dA)));
+                    b[j] = cmul(b[j], cexp_(cmk(0, -p->dt * dB)));
+                }
+                to_k1d(a, n);
+                to_k1d(b, n);
+                for (int j = 0; j < n; j++) { a[j] = cmul(a[j], t.propW[j]); b[j] = cmul(b[j], t.propW[j]); }
+            }
+            if (nn >= 0) {   /* second pulse U^dagger and measurement in k-space (:358-391) */
+                double sa = 0, sb = 0, cre = 0, cim = 0, na = 0, nb = 0;
+                for (int j = 0; j < n; j++) {
+                    orc_c b1 = cscale(cadd(b[j], cmul(cmk(0, 1), a[j])), 1.0 / r2);
+                    orc_c aa = cscale(cadd(a[j], cmul(cmk(0, 1), b[j])), 1.0 / r2);
+                    sa += cnorm(aa);
+                    sb += cnorm(b1);
+                    cre += a[j].re * b[j].re + a[j].im * b[j].im;
+                    cim += a[j].re * b[j].im - a[j].im * b[j].re;
+                    na += cnorm(a[j]);
+                    nb += cnorm(b[j]);
+                }
+                double N1 = n * (sa * p->dz - 0.5 * V);
+                double N2 = n * (sb * p->dz - 0.5 * V);
+                m[nn] += N1; m[nn + ntW] += N2;
+                var[nn] += N1 * N1; var[nn + ntW] += N2 * N2;
+                if (vis) vis[nn] += 2.0 * sqrt(cre * cre + cim * cim) / (na + nb);
+            }
+        }
+        if (avn) {   /* :396-400 */
+            memcpy(a1, a, sizeof(orc_c) * n);
+            orc_dft1d(a1, n, -1);
+            for (int j = 0; j < n; j++) avn[j] += cnorm(a1[j]) - 0.5 * V / p->dz;
+        }
+    }
+    /* :405-413, as written: no absolute value before the square root */
+    for (int i = 0; i < 2 * ntW; i++) {
+        m[i] = m[i] / p->ne;
+        var[i] = var[i] / p->ne - m[i] * m[i];
+        var[i] = sqrt(var[i] / (p->ne - 1));
+    }
+    if (vis) for (int i = 0; i < ntW; i++) vis[i] /= p->ne;
+    if (avn) for (int j = 0; j < n; j++) avn[j] /= p->ne;
+
+    free(a); free(b); free(a1); free(a0); free(da);
+    tables1d_free(&t);
+    return 0;
+}

@@ -175,6 +175,35 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
               double *avn, orc2d_report *rep);
 double orc2d_residual(const orc2d_params *p, const orc_c *psi_x, double *mu_out);
 
+/* ---- 1D CPU program (programs/bec_wigner_2d.cpp/wigner_1d.cpp; the same scheme as the Scilab
+   prototype programs/bec_wigner_1d.scilab/WignerSplitFFT.sce) ---- */
+typedef struct {
+    double zmax, dz;              /* nv = 1 + 2 (int)(zmax / dz) cells at z = -zmax + dz j */
+    double g11, g12, g22, E, mu;
+    double Va, Vb;
+    double dt; int ntG; int itmax;
+    int ntW;                      /* ntW - 1 equilibration steps, then ntW steps each sampled */
+    int ne;
+    double n0;
+    uint64_t seed;
+    double noise_dev;             /* reference: sqrt(1/dz)/2 per quadrature */
+} orc1d_params;
+
+typedef struct {
+    double N0;
+    int nv;
+    double gs_residual;
+    double gs_mu;
+} orc1d_report;
+
+/* DFT of any length by its definition: d[k] = sum_j d[j] exp(sign 2 pi i j k / n) */
+void orc_dft1d(orc_c *d, int n, int sign);
+int orc1d_nv(const orc1d_params *p);
+/* m, var: [2][ntW]; vis: [ntW]; gs_x, avn: nv; ntot: ntG (each may be NULL except m, var) */
+int orc1d_run(const orc1d_params *p, double *m, double *var, double *vis, orc_c *gs_x, double *avn,
+              double *ntot, orc1d_report *rep);
+double orc1d_residual(const orc1d_params *p, const orc_c *psi_x, double *mu_out);
+
 int orc_num_threads(void);
 void orc_set_threads(int n);
 

@@ -67,6 +67,19 @@ class Report2D(C.Structure):
     _fields_ = [("N0", C.c_double), ("gs_iterations", C.c_int), ("gs_residual", C.c_double)]
 
 
+class Params1D(C.Structure):
+    """File-scope parameters of programs/bec_wigner_2d.cpp/wigner_1d.cpp:93-130."""
+    _fields_ = [("zmax", C.c_double), ("dz", C.c_double),
+                ("g11", C.c_double), ("g12", C.c_double), ("g22", C.c_double),
+                ("E", C.c_double), ("mu", C.c_double), ("Va", C.c_double), ("Vb", C.c_double),
+                ("dt", C.c_double), ("ntG", C.c_int), ("itmax", C.c_int), ("ntW", C.c_int), ("ne", C.c_int),
+                ("n0", C.c_double), ("seed", C.c_uint64), ("noise_dev", C.c_double)]
+
+
+class Report1D(C.Structure):
+    _fields_ = [("N0", C.c_double), ("nv", C.c_int), ("gs_residual", C.c_double), ("gs_mu", C.c_double)]
+
+
 RULE_REFERENCE = 0
 RULE_RESIDUAL = 1
 
@@ -121,6 +134,10 @@ def lib():
             "orc_visibility_grid": (C.c_double, [C.POINTER(Calc), _P, _P, C.c_int]),
             "orc2d_run": (C.c_int, [C.POINTER(Params2D), _P, _P, _P, _P, _P, C.POINTER(Report2D)]),
             "orc2d_residual": (C.c_double, [C.POINTER(Params2D), _P, _D]),
+            "orc_dft1d": (None, [_P, C.c_int, C.c_int]),
+            "orc1d_nv": (C.c_int, [C.POINTER(Params1D)]),
+            "orc1d_run": (C.c_int, [C.POINTER(Params1D), _P, _P, _P, _P, _P, _P, C.POINTER(Report1D)]),
+            "orc1d_residual": (C.c_double, [C.POINTER(Params1D), _P, _D]),
             "orc_num_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
         }
@@ -422,6 +439,52 @@ def run2d(p, want_gs=True):
 def residual2d(p, psi):
     mu = C.c_double()
     r = lib().orc2d_residual(C.byref(p), _ptr(_c128(psi)), C.byref(mu))
+    return r, mu.value
+
+
+def dft1d(x, sign):
+    """The length-n DFT by its definition, sum_j x_j exp(sign 2 pi i j k / n)."""
+    d = _c128(np.array(x, dtype=np.complex128))
+    lib().orc_dft1d(_ptr(d), d.size, int(sign))
+    return d
+
+
+def params1d(**kw):
+    """1D program defaults (programs/bec_wigner_2d.cpp/wigner_1d.cpp:93-130); tmaxGP / tmaxWig
+    give ntG = 1 + (int)(tmaxGP/dt), ntW = 1 + (int)(tmaxWig/dt) as the program does."""
+    d = dict(zmax=20.0, dz=1.0, g11=6.18, g12=6.03, g22=5.88, E=2.5, mu=26.8, Va=0.5, Vb=1.0, dt=0.01,
+             ntG=None, itmax=3, ntW=None, ne=20, n0=1.0, seed=0x5EED, noise_dev=None, tmaxGP=1.0, tmaxWig=30.0)
+    d.update(kw)
+    tG, tW = d.pop("tmaxGP"), d.pop("tmaxWig")
+    if d["ntG"] is None:
+        d["ntG"] = 1 + int(tG / d["dt"])
+    if d["ntW"] is None:
+        d["ntW"] = 1 + int(tW / d["dt"])
+    if d["noise_dev"] is None:
+        d["noise_dev"] = np.sqrt(1.0 / d["dz"]) / 2.0
+    return Params1D(**d)
+
+
+def nv1d(p):
+    return lib().orc1d_nv(C.byref(p))
+
+
+def run1d(p):
+    nv, nt = nv1d(p), p.ntW
+    m = np.zeros(2 * nt)
+    var = np.zeros(2 * nt)
+    vis = np.zeros(nt)
+    gs = np.empty(nv, dtype=np.complex128)
+    avn = np.empty(nv)
+    ntot = np.empty(p.ntG)
+    rep = Report1D()
+    lib().orc1d_run(C.byref(p), _ptr(m), _ptr(var), _ptr(vis), _ptr(gs), _ptr(avn), _ptr(ntot), C.byref(rep))
+    return dict(m=m.reshape(2, nt), var=var.reshape(2, nt), vis=vis, gs=gs, avn=avn, ntot=ntot, report=rep)
+
+
+def residual1d(p, psi):
+    mu = C.c_double()
+    r = lib().orc1d_residual(C.byref(p), _ptr(_c128(psi)), C.byref(mu))
     return r, mu.value
 
 

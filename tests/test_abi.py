@@ -108,6 +108,18 @@ def test_invalid_arguments_rejected_on_cpu():
     assert L.bec_run_2d(C.byref(p2), m2, v2, None, None, None, None) == api.ERR_SHAPE
     p2 = bec_amd.params2d(gs_rule=2)
     assert L.bec_run_2d(C.byref(p2), m2, v2, None, None, None, None) == api.ERR_INVALID
+    # the 1D program: lattice size from zmax / dz (the program's 41), limits and bad constants
+    assert L.bec_1d_cells(C.byref(bec_amd.params1d())) == 41
+    assert L.bec_1d_cells(C.byref(bec_amd.params1d(zmax=10.0, dz=0.3))) == 67
+    assert L.bec_1d_cells(C.byref(bec_amd.params1d(zmax=300.0, dz=1.0))) == -api.ERR_SHAPE
+    assert L.bec_1d_cells(C.byref(bec_amd.params1d(dz=0.0, noise_dev=0.5))) == -api.ERR_INVALID
+    assert L.bec_1d_cells(None) == -api.ERR_INVALID
+    m1, v1 = (C.c_double * 2)(), (C.c_double * 2)()
+    for bad in (dict(zmax=300.0), dict(dt=0.0, ntG=101, ntW=3001), dict(ntG=0), dict(itmax=-1), dict(ntW=0), dict(ne=0),
+                dict(ne=100000, ntW=1000)):
+        want = api.ERR_SHAPE if ("zmax" in bad or bad.get("ne") == 100000) else api.ERR_INVALID
+        assert L.bec_run_1d(C.byref(bec_amd.params1d(**bad)), m1, v1, None, None, None, None, None) == want, bad
+    assert L.bec_run_1d(C.byref(bec_amd.params1d()), None, v1, None, None, None, None, None) == api.ERR_INVALID
     # context-less calls
     assert L.bec_propagate(None, 1, 0.0, 0, None) == api.ERR_INVALID
     assert L.bec_observables(None, 0, None) == api.ERR_INVALID

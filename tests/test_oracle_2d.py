@@ -82,3 +82,15 @@ def test_noise_and_vacuum_subtraction_of_the_ensemble(oracle, Va, Vb):
     assert e1 > 0 and e2 > 0
     assert abs(N1 - exp1) < 4 * e1, (N1, exp1, e1)
     assert abs(N2 - exp2) < 4 * e2, (N2, exp2, e2)
+
+
+def test_ground_state_is_stationary_in_real_time(oracle):
+    # the 2D counterpart: with g = 0 the lattice ground state keeps its density
+    # (|a|^2 = |psi_0|^2 / 2 after U(pi/2)) through the real-time steps, which a
+    # flipped kinetic or trap sign would not
+    s = 32 / 31
+    p = oracle.params2d(nvx=32, nvy=32, xmax=10.0, ymax=10.0, g11=0.0, g12=0.0, g22=0.0, E=0.0, mu=s, dtG=0.02,
+                        ntG=2000, ne=1, n_meas=1, ntW=60, dtW=0.01, Va=0.0, Vb=0.0)
+    out = oracle.run2d(p)
+    d0 = np.abs(out["gs"]) ** 2
+    assert np.abs(out["avn"] - d0 / 2).max() < 1e-4 * d0.max()
