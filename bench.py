@@ -138,6 +138,35 @@ def program_2d(B, with_cpu):
     return out
 
 
+def program_1d(B, with_cpu):
+    """The 1D program (programs/bec_wigner_2d.cpp/wigner_1d.cpp) as it stands: 41 cells, 100
+    ground-state steps, 3000 equilibration steps and 3001 sampled steps for 20 trajectories, on the
+    GPU (one wavefront per trajectory) and, for comparison, the FP64 oracle on one host core; plus
+    the same run for 4096 trajectories on the GPU."""
+    B.run_1d(B.params1d(ntW=10, ne=2))                       # first-call setup
+    t0 = time.perf_counter()
+    one = B.run_1d(B.params1d())
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    big = B.run_1d(B.params1d(ne=4096))
+    t_big = time.perf_counter() - t0
+    steps = 2 * 3001 - 1
+    out = {"workload": "41 cells complex128, 100 ground-state steps, 3000 + 3001 steps; 20 trajectories "
+                       "(and 4096)",
+           "gpu_ms_20traj": t_gpu * 1e3, "gpu_device_ms_20traj": one["report"]["elapsed_ms"],
+           "gpu_ms_4096traj": t_big * 1e3, "traj_steps_per_s_4096": 4096 * steps / t_big,
+           "N1_N2_last_sample": [float(one["m"][0, -1]), float(one["m"][1, -1])]}
+    if with_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        ref = O.run1d(O.params1d())
+        t_cpu = time.perf_counter() - t0
+        out.update(cpu_ms_20traj=t_cpu * 1e3, cpu_threads=1, cpu_kind="port",
+                   max_rel_diff_N1_plus_N2=float(np.max(np.abs(one["m"].sum(0) - ref["m"].sum(0)))
+                                                 / ref["m"].sum(0)[0]))
+    return out
+
+
 def two_component_ground_state(B):
     """Two-component ground state (N/2 per component) at the headline parameters, gradient flow
     from the Thomas-Fermi start to residual 1e-6, FP64 (second solve timed)."""
@@ -273,6 +302,8 @@ def main():
     log("two-component ground state done")
     p2d = program_2d(B, not a.no_cpu)
     log("2D program done")
+    p1d = program_1d(B, not a.no_cpu)
+    log("1D program done")
 
     cpu = None
     if not a.no_cpu:
@@ -300,6 +331,7 @@ def main():
         "surrogate_start": surrogate_summary(),
         "uneven_groups": uneven,
         "program_2d": p2d,
+        "program_1d": p1d,
         "final_observables": {k: obs[k] for k in ("t", "N_total", "visibility", "visibility_grid")},
         "n_samples": int(samples.shape[0]),
     }
