@@ -48,3 +48,34 @@ __device__ __forceinline__ void noise4(uint32_t cell, uint32_t traj, uint32_t st
 }
 
 } // namespace bec
+
+namespace bec {
+
+// Four standard normal deviates for the loss noise of step `step`: Philox
+// counter {cell, traj, step, 1 + group}, Box-Muller on 24-bit uniforms
+// u = ((x >> 8) + 1) 2^-24, which FP32 represents exactly, so the FP32 and
+// FP64 paths start from the same uniforms.
+template <typename T>
+__device__ __forceinline__ void loss_noise4(uint32_t cell, uint32_t traj, uint32_t step, uint32_t group, uint32_t k0,
+                                            uint32_t k1, T z[4])
+{
+    uint32_t c[4] = {cell, traj, step, 1u + group};
+    philox4x32_10(c, k0, k1);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        const T u1 = (T)((c[2 * p] >> 8) + 1u) * (T)(1.0 / 16777216.0);
+        const T u2 = (T)((c[2 * p + 1] >> 8) + 1u) * (T)(1.0 / 16777216.0);
+        T rad, sn, cs;
+        if constexpr (sizeof(T) == 4) {
+            rad = sqrtf(-2.0f * __logf(u1));
+            sincospif(2.0f * u2, &sn, &cs);
+        } else {
+            rad = sqrt(-2.0 * log(u1));
+            sincospi(2.0 * u2, &sn, &cs);
+        }
+        z[2 * p] = rad * cs;
+        z[2 * p + 1] = rad * sn;
+    }
+}
+
+} // namespace bec

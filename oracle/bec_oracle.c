@@ -1207,3 +1207,103 @@ int orc1d_run(const orc1d_params *p, double *m, double *var, double *vis, orc_c 
     tables1d_free(&t);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Loss-induced Wigner noise during the evolution                       */
+
+/* Four standard normal deviates for the loss noise of step `step`: Philox
+ * counter {cell, traj, step, 1 + group}, key = seed, Box-Muller on 24-bit
+ * uniforms u = ((x >> 8) + 1) 2^-24. */
+void orc_loss_noise_cell(uint64_t seed, uint32_t traj, uint32_t cell, uint32_t step, uint32_t group, double out[4])
+{
+    uint32_t ctr[4] = {cell, traj, step, 1u + group};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t r[4];
+    orc_philox4x32_10(ctr, key, r);
+    for (int p = 0; p < 2; p++) {
+        double u1 = (double)((r[2 * p] >> 8) + 1u) / 16777216.0;
+        double u2 = (double)((r[2 * p + 1] >> 8) + 1u) / 16777216.0;
+        double rad = sqrt(-2.0 * log(u1));
+        out[2 * p] = rad * cos(2.0 * M_PI * u2);
+        out[2 * p + 1] = rad * sin(2.0 * M_PI * u2);
+    }
+}
+
+/* The Ito increment of the loss noise for one cell, from the state a0, b0 at
+ * the start of the step.  The truncated Wigner equation of the report
+ * (docs/wigner_simulation.report/report.tex:624-656; poster
+ * docs/icap2010.poster/poster.tex:245-290), at the order of the drift the
+ * kernels integrate (-(l111 n1^2 + l12 n2)/2 and -(l12 n1 + l22 n2)/2),
+ * diffuses with
+ *   D11 = 3/2 l111 n1^2 + 1/2 l12 n2,  D22 = 1/2 l12 n1 + l22 n2,
+ *   D12 = 1/2 l12 a0 conj(b0),
+ * i.e. <d psi_i d psi_j*> = D_ij dt / dV per cell.  The increment is B dW with
+ * B the lower Cholesky factor of D and dW_k = (z_2k + i z_2k+1) sqrt(dt/(2 dV)). */
+static void loss_increment(double l111, double l12, double l22, orc_c a0, orc_c b0, const double z[4], double w,
+                           orc_c *da, orc_c *db)
+{
+    double n1 = cnorm(a0), n2 = cnorm(b0);
+    double D11 = 1.5 * l111 * n1 * n1 + 0.5 * l12 * n2;
+    double D22 = 0.5 * l12 * n1 + l22 * n2;
+    orc_c w1 = cmk(z[0] * w, z[1] * w), w2 = cmk(z[2] * w, z[3] * w);
+    double b11 = sqrt(D11), b22;
+    orc_c b21 = cmk(0.0, 0.0);
+    if (D11 > 0) {
+        /* conj(D12) / b11 = l12 conj(a0) b0 / (2 b11) */
+        orc_c cab = cmk(a0.re * b0.re + a0.im * b0.im, a0.re * b0.im - a0.im * b0.re);
+        b21 = cscale(cab, 0.5 * l12 / b11);
+        double r = D22 - 0.25 * l12 * l12 * n1 * n2 / D11;
+        b22 = sqrt(r > 0 ? r : 0.0);
+    } else {
+        b22 = sqrt(D22);
+    }
+    *da = cscale(w1, b11);
+    *db = cadd(cmul(b21, w1), cscale(w2, b22));
+}
+
+void orc_nonlinear_rt_noise(const orc_calc *c, orc_c *aa, orc_c *bb, int ne, double dt, uint64_t seed,
+                            uint32_t step, uint32_t group)
+{
+    long n = (long)c->cells * ne;
+    double w = sqrt(dt / (2.0 * c->dx * c->dy * c->dz));
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        double V = orc_potential(c, (int)(i % c->cells));
+        orc_c a0 = aa[i], b0 = bb[i], a = a0, b = b0;
+        orc_c fa = cmk(0, 0), fb = cmk(0, 0), da, db;
+        /* the deterministic step of orc_nonlinear_rt */
+        for (int it = 0; it < c->itmax; it++) {
+            double na = cnorm(a), nb = cnorm(b);
+            orc_c pa = cmk(-(c->l111 * na * na + c->l12 * nb) / 2.0, V + c->g11 * na + c->g12 * nb);
+            orc_c pb = cmk(-(c->l22 * nb + c->l12 * na) / 2.0, V + c->g22 * nb + c->g12 * na - c->detuning);
+            fa = cexp_(cscale(pa, dt / 2.0));
+            fb = cexp_(cscale(pb, dt / 2.0));
+            a = cmul(a0, fa);
+            b = cmul(b0, fb);
+        }
+        /* plus the noise increment evaluated at the start of the step */
+        double z[4];
+        orc_loss_noise_cell(seed, (uint32_t)(i / c->cells), (uint32_t)(i % c->cells), step, group, z);
+        loss_increment(c->l111, c->l12, c->l22, a0, b0, z, w, &da, &db);
+        aa[i] = cadd(cmul(a, fa), da);
+        bb[i] = cadd(cmul(b, fb), db);
+    }
+}
+
+/* propagate with the loss noise: steps step0, step0 + 1, ... */
+void orc_propagate_noise(const orc_calc *c, orc_c *a, orc_c *b, int ne, double dt, int n_steps, uint64_t seed,
+                         uint32_t step0, uint32_t group)
+{
+    double *k2 = malloc(sizeof(double) * c->cells);
+    orc_ktable(c, k2);
+    for (int s = 0; s < n_steps; s++) {
+        orc_kinetic_rt(c, k2, a, b, ne, dt);
+        orc_to_xspace(c, a, ne);
+        orc_to_xspace(c, b, ne);
+        orc_nonlinear_rt_noise(c, a, b, ne, dt, seed, step0 + (uint32_t)s, group);
+        orc_to_kspace(c, a, ne);
+        orc_to_kspace(c, b, ne);
+        orc_kinetic_rt(c, k2, a, b, ne, dt);
+    }
+    free(k2);
+}

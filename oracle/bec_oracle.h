@@ -175,6 +175,16 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
               double *avn, orc2d_report *rep);
 double orc2d_residual(const orc2d_params *p, const orc_c *psi_x, double *mu_out);
 
+/* ---- loss-induced Wigner noise during the evolution (derived in
+   docs/wigner_simulation.report/report.tex:624-656, not implemented by the reference) ---- */
+/* four standard normal deviates: Philox counter {cell, traj, step, 1 + group} */
+void orc_loss_noise_cell(uint64_t seed, uint32_t traj, uint32_t cell, uint32_t step, uint32_t group, double out[4]);
+/* orc_nonlinear_rt plus the Ito increment B dW of the loss diffusion */
+void orc_nonlinear_rt_noise(const orc_calc *c, orc_c *a_x, orc_c *b_x, int ne, double dt, uint64_t seed,
+                            uint32_t step, uint32_t group);
+void orc_propagate_noise(const orc_calc *c, orc_c *a, orc_c *b, int ne, double dt, int n_steps, uint64_t seed,
+                         uint32_t step0, uint32_t group);
+
 /* ---- 1D CPU program (programs/bec_wigner_2d.cpp/wigner_1d.cpp; the same scheme as the Scilab
    prototype programs/bec_wigner_1d.scilab/WignerSplitFFT.sce) ---- */
 typedef struct {

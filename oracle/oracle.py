@@ -124,6 +124,11 @@ def lib():
             "orc_to_xspace": (None, [C.POINTER(Calc), _P, C.c_int]),
             "orc_init_evolution": (None, [C.POINTER(Calc), _P, C.c_uint64, C.c_int, _P, _P]),
             "orc_propagate": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_int]),
+            "orc_loss_noise_cell": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+            "orc_nonlinear_rt_noise": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_uint64,
+                                              C.c_uint32, C.c_uint32]),
+            "orc_propagate_noise": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_int, C.c_uint64,
+                                           C.c_uint32, C.c_uint32]),
             "orc_kinetic_rt": (None, [C.POINTER(Calc), _P, _P, _P, C.c_int, C.c_double]),
             "orc_nonlinear_rt": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double]),
             "orc_half_pi_pulse": (None, [_P, _P, C.c_long]),
@@ -349,6 +354,28 @@ def propagate(c, a_k, b_k, dt, n_steps):
     ne = a.size // c.cells
     lib().orc_propagate(C.byref(c), _ptr(a), _ptr(b), ne, dt, n_steps)
     return a, b
+
+
+def propagate_noise(c, a_k, b_k, dt, n_steps, seed, step0=0, group=0):
+    """propagate with the loss-induced noise increments of steps step0 .. step0 + n_steps - 1."""
+    a = _c128(a_k).copy()
+    b = _c128(b_k).copy()
+    ne = a.size // c.cells
+    lib().orc_propagate_noise(C.byref(c), _ptr(a), _ptr(b), ne, dt, n_steps, seed, step0, group)
+    return a, b
+
+
+def nonlinear_rt_noise(c, a_x, b_x, dt, seed, step, group=0):
+    a = _c128(a_x).copy()
+    b = _c128(b_x).copy()
+    lib().orc_nonlinear_rt_noise(C.byref(c), _ptr(a), _ptr(b), a.size // c.cells, dt, seed, step, group)
+    return a, b
+
+
+def loss_noise_cell(seed, traj, cell, step, group=0):
+    out = np.zeros(4)
+    lib().orc_loss_noise_cell(seed, traj, cell, step, group, _ptr(out))
+    return out
 
 
 def nonlinear_rt(c, a_x, b_x, dt):
