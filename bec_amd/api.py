@@ -174,6 +174,8 @@ EXPORTS = {
     "bec_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "bec_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bec_set_graphs": (C.c_int, [_P, C.c_int]),
+    "bec_set_loss_noise": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint]),
+    "bec_get_loss_noise_step": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint)]),
     "bec_set_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
     "bec_get_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
     "bec_schedule_candidates": (C.c_int, [_P, C.c_int, C.POINTER(Schedule), C.c_int, C.POINTER(C.c_int)]),
@@ -502,6 +504,16 @@ class Solver:
 
     def set_graphs(self, enable):
         check(lib().bec_set_graphs(self._ctx, 1 if enable else 0), "bec_set_graphs")
+
+    def set_loss_noise(self, enable=True, seed=0x1055, first_step=0):
+        """Add (or remove) the loss-induced Wigner noise to every nonlinear step (bec_set_loss_noise)."""
+        check(lib().bec_set_loss_noise(self._ctx, 1 if enable else 0, int(seed), int(first_step)),
+              "bec_set_loss_noise")
+
+    def loss_noise_step(self, group=0):
+        st = C.c_uint(0)
+        check(lib().bec_get_loss_noise_step(self._ctx, group, C.byref(st)), "bec_get_loss_noise_step")
+        return st.value
 
     def synchronize(self):
         check(lib().bec_synchronize(self._ctx), "bec_synchronize")

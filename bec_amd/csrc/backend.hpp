@@ -32,6 +32,11 @@ struct DevGroup {
     int xy_minw;         // xy pass: minimum waves per SIMD for register allocation (1 or 5)
     int z_half;          // 1: z-pass LDS exchange in two real-valued rounds (half the LDS)
     int itmax1;          // every record has itmax == 1: straight-line nonlinear step
+    // loss-induced noise (bec_set_loss_noise): the device step counter read by
+    // the nonlinear pass and advanced after it; null when the noise is off
+    unsigned *noise_step;
+    uint32_t noise_k0, noise_k1;   // Philox key
+    int group_index;               // keeps the groups' noise streams apart
 };
 
 // One schedule candidate of the hot passes.

@@ -216,6 +216,9 @@ void group_release(Group &g)
     void *r = g.d_sums;
     dev_free(r);
     g.d_sums = nullptr;
+    void *n = d.noise_step;
+    dev_free(n);
+    d.noise_step = nullptr;
 }
 
 int upload_complex(const double *h, void *d, long n, int prec, hipStream_t s)
@@ -960,6 +963,43 @@ const bec_calc_params *ctx_group_params(bec_ctx *ctx, int group, hipStream_t *st
     return &ctx->groups[group].calc;
 }
 } // namespace bec
+
+extern "C" int bec_set_loss_noise(bec_ctx *ctx, int enable, uint64_t seed, unsigned first_step)
+{
+    CHECK_CTX(ctx);
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ctx->groups.size(); ++i) {
+        DevGroup &d = ctx->groups[i].dev;
+        if (!enable) {
+            void *q = d.noise_step;
+            dev_free(q);
+            d.noise_step = nullptr;
+            continue;
+        }
+        if (!d.noise_step) BEC_HIP(dev_alloc((void **)&d.noise_step, sizeof(unsigned)));
+        BEC_HIP(hipMemcpy(d.noise_step, &first_step, sizeof(unsigned), hipMemcpyHostToDevice));
+        d.noise_k0 = (uint32_t)seed;
+        d.noise_k1 = (uint32_t)(seed >> 32);
+        d.group_index = (int)i;
+    }
+    // a captured step graph holds the launches without (or with) the noise
+    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
+    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
+    ctx->body_exec = nullptr;
+    ctx->body_graph = nullptr;
+    return BEC_OK;
+}
+
+extern "C" int bec_get_loss_noise_step(bec_ctx *ctx, int group, unsigned *step)
+{
+    CHECK_GROUP(ctx, group);
+    if (!step) return BEC_ERR_INVALID;
+    const DevGroup &d = ctx->groups[group].dev;
+    if (!d.noise_step) return BEC_ERR_STATE;
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    BEC_HIP(hipMemcpy(step, d.noise_step, sizeof(unsigned), hipMemcpyDeviceToHost));
+    return BEC_OK;
+}
 
 extern "C" int bec_set_graphs(bec_ctx *ctx, int enable)
 {

@@ -128,9 +128,34 @@ hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s)
     });
 }
 
+// Advances the loss-noise step counter after the pass that read it (one
+// thread, a vector store; captured into step graphs like the passes).
+__global__ void noise_tick_kernel(unsigned *step)
+{
+    if (threadIdx.x == 0) step[0] = step[0] + 1u;
+}
+
 template <typename T, int N, bool IT1>
 static hipError_t launch_nonlinear(const DevGroup &g, hipStream_t s)
 {
+    if (g.noise_step) {
+        OpNonlinRT<T, N, IT1, true> op;
+        op.a = (cpx<T> *)g.a;
+        op.b = (cpx<T> *)g.b;
+        op.P = g.params;
+        op.rec_shift = g.rec_shift;
+        op.step = g.noise_step;
+        op.k0 = g.noise_k0;
+        op.k1 = g.noise_k1;
+        op.group = (uint32_t)g.group_index;
+        hipError_t e = with_bool(g.nt, [&](auto NTt) -> hipError_t {
+            constexpr bool NT = decltype(NTt)::value;
+            return launch_xy<T, N, OpNonlinRT<T, N, IT1, true>, NT, 1>(g, op, g.ntraj, nullptr, s, g.xcd);
+        });
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(noise_tick_kernel, dim3(1), dim3(64), 0, s, g.noise_step);
+        return hipGetLastError();
+    }
     OpNonlinRT<T, N, IT1> op;
     op.a = (cpx<T> *)g.a;
     op.b = (cpx<T> *)g.b;

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "philox.hpp"
 #include "fastmath.hpp"
 #include "fft_reg.hpp"
 
@@ -192,14 +193,45 @@ struct SkipDone {
 // form for itmax = 1 (the reference default, wigner.cpp:90): densities from
 // the input, d = exp(p dt/2), output psi0 d d.  IT1 = false runs the general
 // midpoint iteration with a runtime trip count.
-template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
+//
+// NOISE adds the loss-induced Wigner noise that the reference derives
+// (docs/wigner_simulation.report/report.tex:624-656) but does not integrate:
+// the Ito increment B dW of the leading-order loss diffusion, evaluated at the
+// input state, with the Philox stream {cell, traj, step, 1 + group}.
+template <typename T, int N, bool IT1 = true, bool NOISE = false> struct OpNonlinRT : NoSkip {
     static constexpr int NC = 2, SIGN_IN = -1, SIGN_OUT = +1, NSUM = 0;
     static constexpr bool WRITE = true;
     cpx<T> *a, *b;
     const TrajParams *P;
     int rec_shift;
+    const unsigned *step;          // NOISE: the step counter
+    uint32_t k0, k1, group;        // NOISE: Philox key, group index
     __device__ const cpx<T> *src(int c) const { return c == 0 ? a : b; }
     __device__ cpx<T> *dst(int c) const { return c == 0 ? a : b; }
+    // B dW for one cell: D11 = 3/2 l111 n1^2 + 1/2 l12 n2, D22 = 1/2 l12 n1 + l22 n2,
+    // D12 = 1/2 l12 a0 conj(b0), <d psi_i d psi_j*> = D_ij dt/dV; B is the lower
+    // Cholesky factor and dW_k = (z_2k + i z_2k+1) sqrt(dt / (2 dV)), times `scale`.
+    static __device__ __forceinline__ void loss_increment(T l111, T l12, T l22, cpx<T> a0, cpx<T> b0, const T z[4],
+                                                          T ws, cpx<T> &da, cpx<T> &db)
+    {
+        const T n1 = cnorm<T>(a0), n2 = cnorm<T>(b0);
+        const T D11 = (T)1.5 * l111 * n1 * n1 + (T)0.5 * l12 * n2;
+        const T D22 = (T)0.5 * l12 * n1 + l22 * n2;
+        const cpx<T> w1 = mk<T>(z[0] * ws, z[1] * ws), w2 = mk<T>(z[2] * ws, z[3] * ws);
+        const T b11 = sqrt_t(D11);
+        cpx<T> b21 = mk<T>(0, 0);
+        T b22;
+        if (D11 > (T)0) {
+            const cpx<T> cab = mk<T>(a0.x * b0.x + a0.y * b0.y, a0.x * b0.y - a0.y * b0.x);   // conj(a0) b0
+            b21 = cscale<T>(cab, (T)0.5 * l12 / b11);
+            const T r = D22 - (T)0.25 * l12 * l12 * n1 * n2 / D11;
+            b22 = sqrt_t(r > (T)0 ? r : (T)0);
+        } else {
+            b22 = sqrt_t(D22);
+        }
+        da = cscale<T>(w1, b11);
+        db = mk<T>(b21.x * w1.x - b21.y * w1.y + b22 * w2.x, b21.x * w1.y + b21.y * w1.x + b22 * w2.y);
+    }
     __device__ __forceinline__ void apply(cpx<T> (&r)[2][N], int y, int z, int traj, long, double *) const
     {
         const TrajParams &p = P[traj >> rec_shift];
@@ -211,6 +243,24 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
         const T yy = (T)(-p.ymax) + (T)p.dy * (T)y;
         const T zz = (T)(-p.zmax) + (T)p.dz * (T)z;
         const T vyz = yy * yy + zz * zz * (T)p.inv_lambda2;
+        // the loss noise of this row: the step, dW scaled by the x->k share 1/(NX NY)
+        uint32_t st = 0;
+        T ws = (T)0;
+        if constexpr (NOISE) {
+            st = *step;
+            ws = (T)(sqrt(p.dt / (2.0 * p.dx * p.dy * p.dz)) / (N * N));
+        }
+        const uint32_t cell0 = (uint32_t)((z * N + y) * N);
+        auto add_noise = [&](int x, cpx<T> a0, cpx<T> b0) {
+            if constexpr (NOISE) {
+                T zn[4];
+                loss_noise4<T>(cell0 + (uint32_t)x, (uint32_t)traj, st, group, k0, k1, zn);
+                cpx<T> da, db;
+                loss_increment(l111, l12, l22, a0, b0, zn, ws, da, db);
+                r[0][x] = mk<T>(r[0][x].x + da.x, r[0][x].y + da.y);
+                r[1][x] = mk<T>(r[1][x].x + db.x, r[1][x].y + db.y);
+            }
+        };
         if constexpr (IT1) {
             // itmax = 1: d = exp(p(psi0) dt/2) and the output psi0 d^2, i.e. the
             // phase 2 phi = Im(p) dt and the loss exponent y = Re(p) dt, with dt
@@ -237,6 +287,7 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
                 const cpx<T> vb = cscale<T>(cmul<T>(b0, mk<T>(cb, sb)), scale);
                 r[0][x] = mk<T>(fma_t(va.x, ta, va.x), fma_t(va.y, ta, va.y));
                 r[1][x] = mk<T>(fma_t(vb.x, tb, vb.x), fma_t(vb.y, tb, vb.y));
+                add_noise(x, a0, b0);
             });
             return;
         }
@@ -273,6 +324,7 @@ template <typename T, int N, bool IT1 = true> struct OpNonlinRT : NoSkip {
             const cpx<T> vb = cscale<T>(cmul<T>(b0, cmul<T>(eb, eb)), scale);
             r[0][x] = mk<T>(fma_t(va.x, ta, va.x), fma_t(va.y, ta, va.y));
             r[1][x] = mk<T>(fma_t(vb.x, tb, vb.x), fma_t(vb.y, tb, vb.y));
+            add_noise(x, a0, b0);
         });
     }
     // sin and cos of the doubled phase (IT1 path)

@@ -351,6 +351,27 @@ int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *results, int 
  * inside bec_propagate (disabled: one launch per pass). */
 int bec_set_graphs(bec_ctx *ctx, int enable);
 
+/* Loss-induced Wigner noise during the evolution.  No counterpart in the
+ * reference's code: its report derives the diffusion of the loss terms
+ * (docs/wigner_simulation.report/report.tex:624-656; the poster
+ * docs/icap2010.poster/poster.tex:245-290) but its kernel
+ * (wigner_kernel.cu:184-229) integrates only the drift.  enable != 0 adds to
+ * every nonlinear step of every group (bec_propagate, the equilibration of
+ * bec_init_evolution, bec_autotune and bec_time_passes steps alike) the Ito
+ * increment of the leading-order diffusion, at the order of the drift the
+ * kernel integrates:
+ *   D11 = 3/2 l111 |a|^4 + 1/2 l12 |b|^2,  D22 = 1/2 l12 |a|^2 + l22 |b|^2,
+ *   D12 = 1/2 l12 a conj(b),  <d psi_i d psi_j*> = D_ij dt / dV per cell,
+ * from the state at the start of the step, with normal deviates from the
+ * Philox stream {cell, trajectory, step, 1 + group}, key seed.  The step
+ * counter of each group starts at first_step and advances once per nonlinear
+ * step, so a run resumed from a checkpoint continues the same stream when it
+ * passes the step read by bec_get_loss_noise_step.  enable = 0 (the default)
+ * restores the reference's noiseless step. */
+int bec_set_loss_noise(bec_ctx *ctx, int enable, uint64_t seed, unsigned first_step);
+/* The next step index of group `group`'s loss-noise stream (BEC_ERR_STATE when off). */
+int bec_get_loss_noise_step(bec_ctx *ctx, int group, unsigned *step);
+
 /* ------------------------------------------------- batched FFT sub-API */
 /* batchfftPlan3d / batchfftExecute / batchfftDestroy (batchfft.h:83-108).
  * Unnormalised C2C over batch contiguous nz x ny x nx arrays in device
