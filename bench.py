@@ -167,6 +167,32 @@ def program_1d(B, with_cpu):
     return out
 
 
+def loss_noise_cost(B, psi, traj, steps):
+    """The headline workload with the loss-induced Wigner noise switched on against the same run without
+    it (same context, schedule and graphs), plus the xy-pass time at 2048 trajectories with the noise."""
+    c = B.derive(B.default_model(**config1(ne=traj)))
+    out = {"workload": f"{traj} trajectories x 128x16x16, {steps} steps, graphs on"}
+    with B.Solver(c, precision=B.FP32) as s:
+        s.init_evolution(psi, SEED, equilibrate=False)
+        for noise in (False, True):
+            s.set_loss_noise(noise, seed=0x1055)
+            s.propagate(200)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.propagate(steps)
+            s.synchronize()
+            out["ms_per_step_noise" if noise else "ms_per_step_plain"] = (time.perf_counter() - t0) / steps * 1e3
+    cb = B.derive(B.default_model(**config1(ne=2048)))
+    with B.Solver(cb, precision=B.FP32) as s:
+        s.init_evolution(psi, SEED, equilibrate=False)
+        ms = s.time_passes(iters=20)
+        s.set_loss_noise(True, seed=0x1055)
+        msn = s.time_passes(iters=20)
+    out["xy_pass_ms_2048traj"] = {"plain": ms["xy_pass"], "noise": msn["xy_pass"]}
+    out["noise_over_plain"] = out["ms_per_step_noise"] / out["ms_per_step_plain"]
+    return out
+
+
 def two_component_ground_state(B):
     """Two-component ground state (N/2 per component) at the headline parameters, gradient flow
     from the Thomas-Fermi start to residual 1e-6, FP64 (second solve timed)."""
@@ -300,6 +326,8 @@ def main():
     log("uneven-groups leg done")
     gs2 = two_component_ground_state(B)
     log("two-component ground state done")
+    lnoise = loss_noise_cost(B, psi, a.traj, 2000)
+    log("loss-noise leg done")
     p2d = program_2d(B, not a.no_cpu)
     log("2D program done")
     p1d = program_1d(B, not a.no_cpu)
@@ -328,6 +356,7 @@ def main():
             "reference_rule_ms": rep_ref["elapsed_ms"], "reference_rule_iterations": rep_ref["iterations"],
             "reference_rule_residual": rep_ref["residual"]},
         "ground_state_two_component": gs2,
+        "loss_noise": lnoise,
         "surrogate_start": surrogate_summary(),
         "uneven_groups": uneven,
         "program_2d": p2d,
