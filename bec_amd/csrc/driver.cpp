@@ -181,6 +181,10 @@ int main(int argc, char **argv)
            rep.E_ref, rep.mu_ref, rep.residual);
 
     const uint64_t seed = (uint64_t)num(kv, "seed", 0x5EED);
+    // loss-induced Wigner noise (off by default, as in the reference's kernel);
+    // switched on before the ensembles so that the equilibration steps carry it too
+    if (flag(kv, "loss_noise", false))
+        CHECK(bec_set_loss_noise(ctx, 1, (uint64_t)num(kv, "loss_noise_seed", (double)(seed + 1)), 0));
     CHECK(bec_init_evolution(ctx, 0, gs.data(), seed, flag(kv, "equilibrate", true) ? 1 : 0));
 
     const int steps = (int)num(kv, "steps", 15000);

@@ -76,3 +76,18 @@ def test_driver_writes_reference_schema_and_matches_api(bec, tmp_path):
     assert np.allclose(series["N"][1], [o["N_total"] for o in obs], rtol=1e-10)
     assert np.allclose(series["Na"][1], [o["Na"] / 4 for o in obs], rtol=1e-10)
     assert np.allclose(err, [o["Na_stderr"] for o in obs], rtol=1e-8)
+
+
+def test_driver_loss_noise_option(bec, tmp_path):
+    exe = os.path.join(ROOT, "bec_amd", "bec_run")
+    series = {}
+    for noise in ("false", "true"):
+        cfg = tmp_path / f"exp_{noise}.yaml"
+        cfg.write_text(EXPERIMENT.replace("gamma12: 0.78e-13", "gamma12: 0.78e-11") + f"loss_noise: {noise}\n")
+        prefix = str(tmp_path / f"out_{noise}")
+        r = subprocess.run([exe, str(cfg), prefix], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        with open(f"{prefix}_Na.yaml") as f:
+            series[noise] = np.array(ast.literal_eval(yaml.safe_load(f)["yarray"]))
+    # the noise is on from the equilibration step on, so every sample carries it
+    assert not np.allclose(series["true"], series["false"], rtol=1e-9)
