@@ -113,6 +113,14 @@ class Report1D(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class AvgReport(C.Structure):
+    _fields_ = [("N", C.c_double), ("norm", C.c_double), ("var_re_a", C.c_double), ("var_im_a", C.c_double),
+                ("var_re_b", C.c_double), ("var_im_b", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Obs(C.Structure):
     _fields_ = [("t", C.c_double), ("Na", C.c_double), ("Nb", C.c_double), ("N_total", C.c_double),
                 ("visibility_grid", C.c_double), ("visibility", C.c_double), ("phase", C.c_double),
@@ -175,6 +183,7 @@ EXPORTS = {
     "bec_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bec_set_graphs": (C.c_int, [_P, C.c_int]),
     "bec_set_loss_noise": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint]),
+    "bec_calculate_averages": (C.c_int, [_P, C.c_int, C.POINTER(AvgReport)]),
     "bec_get_loss_noise_step": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint)]),
     "bec_set_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
     "bec_get_schedule": (C.c_int, [_P, C.c_int, C.POINTER(Schedule)]),
@@ -509,6 +518,13 @@ class Solver:
         """Add (or remove) the loss-induced Wigner noise to every nonlinear step (bec_set_loss_noise)."""
         check(lib().bec_set_loss_noise(self._ctx, 1 if enable else 0, int(seed), int(first_step)),
               "bec_set_loss_noise")
+
+    def averages(self, group=0):
+        """calculateAverages (wigner.cu:305-337): atoms per trajectory, the scale and the ensemble
+        variances of the four quadratures (bec_calculate_averages)."""
+        r = AvgReport()
+        check(lib().bec_calculate_averages(self._ctx, group, C.byref(r)), "bec_calculate_averages")
+        return r.as_dict()
 
     def loss_noise_step(self, group=0):
         st = C.c_uint(0)

@@ -58,6 +58,8 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
 hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s);
 hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s);
 hipError_t run_xy_sample(const DevGroup &g, double *partial, hipStream_t s);
+// per-cell ensemble statistics of x-space states ax, bx: out [cells][6]
+hipError_t run_averages(const DevGroup &g, const void *ax, const void *bx, double *out, hipStream_t s);
 hipError_t run_reduce_partials(const double *partial, int per_traj, int ntraj, int nsum,
                                double *out, hipStream_t s);
 // 2D transform of every plane of `ntraj` trajectories (in -> out), times scale

@@ -838,6 +838,47 @@ extern "C" int bec_set_state(bec_ctx *ctx, int group, int space, const double *a
     return BEC_OK;
 }
 
+extern "C" int bec_calculate_averages(bec_ctx *ctx, int group, bec_avg_report *out)
+{
+    CHECK_GROUP(ctx, group);
+    if (!out) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    const long n = g.cells * g.dev.ntraj;
+    void *ax = nullptr, *bx = nullptr;
+    double *d_out = nullptr;
+    struct Free {
+        void **p[3];
+        ~Free()
+        {
+            for (void **q : p) dev_free(*q);
+        }
+    } fr{{&ax, &bx, (void **)&d_out}};
+    BEC_HIP(dev_alloc(&ax, elem_size(g.dev.prec) * n));
+    BEC_HIP(dev_alloc(&bx, elem_size(g.dev.prec) * n));
+    BEC_HIP(dev_alloc((void **)&d_out, sizeof(double) * 6 * g.cells));
+    BEC_HIP(run_xy_fft(g.dev, g.dev.a, ax, g.dev.ntraj, -1, 1.0, ctx->stream));
+    BEC_HIP(run_xy_fft(g.dev, g.dev.b, bx, g.dev.ntraj, -1, 1.0, ctx->stream));
+    BEC_HIP(run_averages(g.dev, ax, bx, d_out, ctx->stream));
+    std::vector<double> h(6 * (size_t)g.cells);
+    BEC_HIP(hipMemcpyAsync(h.data(), d_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (long i = 0; i < g.cells; ++i)
+        for (int k = 0; k < 6; ++k) t[k] += h[6 * i + k];
+    const bec_calc_params &c = g.calc;
+    const int ne = g.dev.ntraj;
+    const double dV = c.dx * c.dy * c.dz;
+    const double norm = (double)c.N / ((t[0] + t[1]) / ne);
+    out->N = (t[0] + t[1]) * dV / ne;
+    out->norm = norm;
+    out->var_re_a = t[2] / ((double)g.cells * ne) * norm;
+    out->var_im_a = t[3] / ((double)g.cells * ne) * norm;
+    out->var_re_b = t[4] / ((double)g.cells * ne) * norm;
+    out->var_im_b = t[5] / ((double)g.cells * ne) * norm;
+    return BEC_OK;
+}
+
 // -------------------------------------------------------------- timing
 
 extern "C" int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms)

@@ -180,6 +180,52 @@ hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s)
     });
 }
 
+// calculateAverages (wigner.cu:305-337) per cell: one thread per cell walks the
+// trajectories twice (mean, then squared deviations, as kernelAverage,
+// wigner_kernel.cu:383-392) in FP64; out [cells][6] = {sum|a|^2, sum|b|^2,
+// dev Re a, dev Im a, dev Re b, dev Im b}.
+template <typename T>
+__global__ __launch_bounds__(256) void averages_kernel(const cpx<T> *a, const cpx<T> *b, long cells, int ntraj,
+                                                       double *out)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= cells) return;
+    double m[4] = {0, 0, 0, 0}, sa = 0.0, sb = 0.0;
+    for (int j = 0; j < ntraj; ++j) {
+        const cpx<T> x = a[(long)j * cells + i], y = b[(long)j * cells + i];
+        m[0] += (double)x.x;
+        m[1] += (double)x.y;
+        m[2] += (double)y.x;
+        m[3] += (double)y.y;
+        sa += (double)x.x * x.x + (double)x.y * x.y;
+        sb += (double)y.x * y.x + (double)y.y * y.y;
+    }
+    for (int k = 0; k < 4; ++k) m[k] /= ntraj;
+    double d[4] = {0, 0, 0, 0};
+    for (int j = 0; j < ntraj; ++j) {
+        const cpx<T> x = a[(long)j * cells + i], y = b[(long)j * cells + i];
+        const double v[4] = {(double)x.x, (double)x.y, (double)y.x, (double)y.y};
+        for (int k = 0; k < 4; ++k) d[k] += (v[k] - m[k]) * (v[k] - m[k]);
+    }
+    double *o = out + i * 6;
+    o[0] = sa;
+    o[1] = sb;
+    for (int k = 0; k < 4; ++k) o[2 + k] = d[k];
+}
+
+hipError_t run_averages(const DevGroup &g, const void *ax, const void *bx, double *out, hipStream_t s)
+{
+    const long cells = (long)g.nx * g.ny * g.nz;
+    const dim3 grid((unsigned)((cells + 255) / 256));
+    if (g.prec == 0)
+        hipLaunchKernelGGL(averages_kernel<float>, grid, dim3(256), 0, s, (const cpx<float> *)ax,
+                           (const cpx<float> *)bx, cells, g.ntraj, out);
+    else
+        hipLaunchKernelGGL(averages_kernel<double>, grid, dim3(256), 0, s, (const cpx<double> *)ax,
+                           (const cpx<double> *)bx, cells, g.ntraj, out);
+    return hipGetLastError();
+}
+
 hipError_t run_xy_sample(const DevGroup &g, double *partial, hipStream_t s)
 {
     return dispatch(g, [&](auto Tt, auto Nt, auto) -> hipError_t {

@@ -347,6 +347,20 @@ int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *out, int max_
 int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *results, int max_results,
                  int *n_results, bec_schedule *best);
 
+/* calculateAverages (wigner.cu:305-337), the reference's debug report of the
+ * ensemble, on the x-space state of group `group` (its per-cell ensemble sums
+ * in fixed order, quirk Q6 fixed):
+ *   N     = (sum|a|^2 + sum|b|^2) dV / ne, the atoms per trajectory it prints;
+ *   norm  = N_model / ((sum|a|^2 + sum|b|^2) / ne);
+ *   var_* = the ensemble variance (1/ne) sum_j (x_j - <x>)^2 of each quadrature
+ *           per cell (kernelAverage, wigner_kernel.cu:383-392), averaged over
+ *           cells and multiplied by norm, as it prints them. */
+typedef struct bec_avg_report {
+    double N, norm;
+    double var_re_a, var_im_a, var_re_b, var_im_b;
+} bec_avg_report;
+int bec_calculate_averages(bec_ctx *ctx, int group, bec_avg_report *out);
+
 /* Enable (default) or disable the replay of hipGraph-captured step blocks
  * inside bec_propagate (disabled: one launch per pass). */
 int bec_set_graphs(bec_ctx *ctx, int enable);

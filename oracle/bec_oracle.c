@@ -1307,3 +1307,36 @@ void orc_propagate_noise(const orc_calc *c, orc_c *a, orc_c *b, int ne, double d
     }
     free(k2);
 }
+
+/* calculateAverages (wigner.cu:305-337, the debug report; its smallReduce of
+ * quirk Q6 fixed): on x-space states, out = {N, norm, var Re a, var Im a,
+ * var Re b, var Im b} with N = (sum|a|^2 + sum|b|^2) dV / ne,
+ * norm = N_model / ((sum|a|^2 + sum|b|^2) / ne), and each variance the
+ * ensemble variance (1/ne) sum_j (x_j - mean)^2 of one quadrature per cell
+ * (kernelAverage, wigner_kernel.cu:383-392), averaged over cells, times norm. */
+void orc_averages(const orc_calc *c, const orc_c *a, const orc_c *b, int ne, double out[6])
+{
+    double sa = 0, sb = 0, v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < c->cells; i++) {
+        double m[4] = {0, 0, 0, 0};
+        for (int e = 0; e < ne; e++) {
+            long idx = (long)e * c->cells + i;
+            m[0] += a[idx].re; m[1] += a[idx].im; m[2] += b[idx].re; m[3] += b[idx].im;
+            sa += cnorm(a[idx]);
+            sb += cnorm(b[idx]);
+        }
+        for (int k = 0; k < 4; k++) m[k] /= ne;
+        double d[4] = {0, 0, 0, 0};
+        for (int e = 0; e < ne; e++) {
+            long idx = (long)e * c->cells + i;
+            double x[4] = {a[idx].re, a[idx].im, b[idx].re, b[idx].im};
+            for (int k = 0; k < 4; k++) d[k] += (x[k] - m[k]) * (x[k] - m[k]);
+        }
+        for (int k = 0; k < 4; k++) v[k] += d[k];
+    }
+    double dV = c->dx * c->dy * c->dz;
+    double norm = (double)c->N / ((sa + sb) / ne);
+    out[0] = (sa + sb) * dV / ne;
+    out[1] = norm;
+    for (int k = 0; k < 4; k++) out[2 + k] = v[k] / ((double)c->cells * ne) * norm;
+}

@@ -175,6 +175,9 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
               double *avn, orc2d_report *rep);
 double orc2d_residual(const orc2d_params *p, const orc_c *psi_x, double *mu_out);
 
+/* calculateAverages (wigner.cu:305-337) on x-space states: {N, norm, var Re a, var Im a, var Re b, var Im b} */
+void orc_averages(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne, double out[6]);
+
 /* ---- loss-induced Wigner noise during the evolution (derived in
    docs/wigner_simulation.report/report.tex:624-656, not implemented by the reference) ---- */
 /* four standard normal deviates: Philox counter {cell, traj, step, 1 + group} */

@@ -124,6 +124,7 @@ def lib():
             "orc_to_xspace": (None, [C.POINTER(Calc), _P, C.c_int]),
             "orc_init_evolution": (None, [C.POINTER(Calc), _P, C.c_uint64, C.c_int, _P, _P]),
             "orc_propagate": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_int]),
+            "orc_averages": (None, [C.POINTER(Calc), _P, _P, C.c_int, _P]),
             "orc_loss_noise_cell": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
             "orc_nonlinear_rt_noise": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_uint64,
                                               C.c_uint32, C.c_uint32]),
@@ -354,6 +355,14 @@ def propagate(c, a_k, b_k, dt, n_steps):
     ne = a.size // c.cells
     lib().orc_propagate(C.byref(c), _ptr(a), _ptr(b), ne, dt, n_steps)
     return a, b
+
+
+def averages(c, a_x, b_x):
+    """calculateAverages (wigner.cu:305-337) on x-space states."""
+    a = _c128(a_x)
+    out = np.zeros(6)
+    lib().orc_averages(C.byref(c), _ptr(a), _ptr(_c128(b_x)), a.size // c.cells, _ptr(out))
+    return dict(zip(("N", "norm", "var_re_a", "var_im_a", "var_re_b", "var_im_b"), out))
 
 
 def propagate_noise(c, a_k, b_k, dt, n_steps, seed, step0=0, group=0):

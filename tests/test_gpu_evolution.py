@@ -211,3 +211,18 @@ def test_propagate_every_lattice_matches_oracle(bec, oracle, nxy, nz, prec, tol)
     eb = rel_l2(bx, obx, axis=(1, 2, 3))
     assert max(ea.max(), eb.max()) < tol, (ea, eb)
     assert abs(obs["visibility_grid"] - oracle.visibility_grid(oc, oax, obx)) < max(tol, 1e-10) * 10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_averages_match_oracle(bec, oracle, gs, prec):
+    # calculateAverages (wigner.cu:305-337) on the GPU against the oracle's on the
+    # same x-space state (the state get_state returns)
+    c, oc = _pair(bec, oracle, prec, ne=5)
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(gs, SEED, equilibrate=True)
+        s.propagate(10)
+        got = s.averages()
+        ax, bx = s.get_state(bec.SPACE_X)
+    ref = oracle.averages(oc, ax, bx)
+    for k, v in ref.items():
+        assert got[k] == pytest.approx(v, rel=1e-11), k
