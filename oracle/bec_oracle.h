@@ -2,9 +2,13 @@
  * FP64 CPU oracle for the split-step Wigner GPE propagator.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
- * algorithm (programs/bec_wigner_3d.cuda and programs/bec_wigner_2d.cpp of
- * fjarri-attic/bec).  It is the checker that the HIP path is compared against
- * and the timed CPU baseline.  Nothing in the product library links to it.
+ * algorithm (programs/bec_wigner_3d.cuda, programs/bec_wigner_2d.cpp/wigner.cpp
+ * and wigner_1d.cpp of fjarri-attic/bec), plus the loss-induced noise its
+ * report derives but its code does not integrate.  It is the checker that
+ * the HIP path is compared against and the timed CPU baseline.  Nothing in
+ * the product library links to it.  Every function is pinned by closed forms,
+ * known answers or exact quantum dynamics in tests/test_oracle_*.py; none is
+ * parity-unpinned.
  *
  * Conventions (identical to the reference):
  *   - cell index = z*(nx*ny) + y*nx + x, x fastest; trajectories outermost
