@@ -132,7 +132,7 @@ class Obs(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("pw", C.c_int), ("z_r1", C.c_int), ("z_split", C.c_int), ("nt", C.c_int), ("xcd", C.c_int),
-                ("xy_minw", C.c_int), ("z_half", C.c_int), ("ms_per_step", C.c_double)]
+                ("xy_minw", C.c_int), ("z_half", C.c_int), ("xy_split", C.c_int), ("ms_per_step", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

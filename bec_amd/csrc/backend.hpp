@@ -31,6 +31,7 @@ struct DevGroup {
     int xcd;             // 1: XCD-aware workgroup remap (needs ntraj % 8 == 0)
     int xy_minw;         // xy pass: minimum waves per SIMD for register allocation (1 or 5)
     int z_half;          // 1: z-pass LDS exchange in two real-valued rounds (half the LDS)
+    int xy_split;        // 1: real-time xy pass with one wavefront per component (FP32, itmax 1, no noise)
     int itmax1;          // every record has itmax == 1: straight-line nonlinear step
     // loss-induced noise (bec_set_loss_noise): the device step counter read by
     // the nonlinear pass and advanced after it; null when the noise is off
@@ -41,7 +42,7 @@ struct DevGroup {
 
 // One schedule candidate of the hot passes.
 struct Schedule {
-    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half;
+    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half, xy_split;
 };
 
 enum { KT_RT_HALF = 0, KT_RT_FULL = 1, KT_IM_HALF = 2, KT_COUNT = 3 };

@@ -177,6 +177,7 @@ int group_setup(Group &g, const bec_calc_params &c, int prec, int ntraj, bool al
     d.xcd = ntraj % 8 == 0 ? 1 : 0;
     d.xy_minw = 1;
     d.z_half = 0;
+    d.xy_split = 0;
     d.itmax1 = c.itmax == 1 ? 1 : 0;
     const size_t es = elem_size(prec);
     if (alloc_state) {
@@ -1070,7 +1071,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     for (int i = 0; i < n && i < 64; ++i)
         if (cand[i].pw == sch->pw && cand[i].z_r1 == sch->z_r1 && cand[i].z_split == sch->z_split &&
             cand[i].nt == sch->nt && cand[i].xcd == sch->xcd && cand[i].xy_minw == sch->xy_minw &&
-            cand[i].z_half == sch->z_half)
+            cand[i].z_half == sch->z_half && cand[i].xy_split == sch->xy_split)
             ok = true;
     if (!ok) return BEC_ERR_UNSUPPORTED;
     g.dev.pw = sch->pw;
@@ -1080,6 +1081,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     g.dev.xcd = sch->xcd;
     g.dev.xy_minw = sch->xy_minw;
     g.dev.z_half = sch->z_half;
+    g.dev.xy_split = sch->xy_split;
     // a captured step graph holds the old launch configuration
     if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
     if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
@@ -1100,6 +1102,7 @@ extern "C" int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch)
     sch->xcd = d.xcd;
     sch->xy_minw = d.xy_minw ? d.xy_minw : 1;
     sch->z_half = d.z_half;
+    sch->xy_split = d.xy_split;
     sch->ms_per_step = 0;
     return BEC_OK;
 }
@@ -1119,6 +1122,7 @@ extern "C" int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *ou
         out[i].xcd = cand[i].xcd;
         out[i].xy_minw = cand[i].xy_minw;
         out[i].z_half = cand[i].z_half;
+        out[i].xy_split = cand[i].xy_split;
         out[i].ms_per_step = 0;
     }
     *count = n;
@@ -1161,6 +1165,7 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
         g.dev.xcd = cand[i].xcd;
         g.dev.xy_minw = cand[i].xy_minw;
         g.dev.z_half = cand[i].z_half;
+        g.dev.xy_split = cand[i].xy_split;
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {   // first repetition warms up
             e = hipEventRecord(ctx->ev0, ctx->stream);
@@ -1187,6 +1192,7 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
             results[i].xcd = cand[i].xcd;
             results[i].xy_minw = cand[i].xy_minw;
             results[i].z_half = cand[i].z_half;
+            results[i].xy_split = cand[i].xy_split;
             results[i].ms_per_step = per;
         }
         if (per < best_ms) {
@@ -1197,7 +1203,8 @@ extern "C" int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *re
     g.dev = saved;
     if (st == BEC_OK && best_i >= 0) {
         bec_schedule b{cand[best_i].pw, cand[best_i].z_r1, cand[best_i].z_split, cand[best_i].nt,
-                       cand[best_i].xcd, cand[best_i].xy_minw, cand[best_i].z_half, best_ms};
+                       cand[best_i].xcd, cand[best_i].xy_minw, cand[best_i].z_half, cand[best_i].xy_split,
+                       best_ms};
         st = bec_set_schedule(ctx, group, &b);
         if (best) *best = b;
     }

@@ -57,6 +57,12 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
                 for (int half = 0; half < 2; ++half)
                     for (int xcd = 0; xcd < xcds; ++xcd)
                         if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcd, 1, half});
+    // the split xy pass (one wavefront per component) on the XCD-aware family
+    for (int pw : {32, 64})
+        for (int r1 : r1s)
+            for (int nt = 0; nt < 2; ++nt)
+                for (int half = 0; half < 2; ++half)
+                    if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcds - 1, 1, half, 1});
     // the xy occupancy target on the streaming, XCD-aware family
     for (int pw : {32, 64})
         for (int half = 0; half < 2; ++half)
@@ -164,6 +170,14 @@ static hipError_t launch_nonlinear(const DevGroup &g, hipStream_t s)
     return with_bool(g.nt, [&](auto NTt) -> hipError_t {
         constexpr bool NT = decltype(NTt)::value;
         if constexpr (sizeof(T) == 4 && IT1) {
+            if (g.xy_split) {
+                constexpr int PPW = 64 / N;
+                XyGeom geo{g.nz, (long)g.ntraj * g.nz, (g.xcd && g.ntraj % 8 == 0) ? 1 : 0};
+                if (geo.n_planes % PPW) return hipErrorInvalidValue;
+                hipLaunchKernelGGL((xy_split_kernel<N, NT>), dim3((unsigned)(geo.n_planes / PPW)), dim3(128), 0, s,
+                                   op, geo);
+                return hipGetLastError();
+            }
             if (g.xy_minw == 5) return launch_xy<T, N, OpNonlinRT<T, N, IT1>, NT, 5>(g, op, g.ntraj, nullptr, s, g.xcd);
         }
         return launch_xy<T, N, OpNonlinRT<T, N, IT1>, NT, 1>(g, op, g.ntraj, nullptr, s, g.xcd);

@@ -327,6 +327,38 @@ template <typename T, int N, bool IT1 = true, bool NOISE = false> struct OpNonli
             add_noise(x, a0, b0);
         });
     }
+    // The IT1 step of one component c (0: a, 1: b) of a row, given the other
+    // component's densities m (xy_split_kernel): the same operations, in the
+    // same order, as apply() performs for that component.
+    __device__ __forceinline__ void apply_one(cpx<T> (&r)[N], const T (&m)[N], int comp, int y, int z,
+                                              int traj) const
+    {
+        static_assert(IT1 && !NOISE, "the split pass carries the straight-line noiseless step");
+        const TrajParams &p = P[traj >> rec_shift];
+        const double dt = p.dt;
+        const T half_dt = (T)(0.5 * dt);
+        const T scale = (T)(1.0 / (N * N));
+        const T yy = (T)(-p.ymax) + (T)p.dy * (T)y;
+        const T zz = (T)(-p.zmax) + (T)p.dz * (T)z;
+        const T vyz = yy * yy + zz * zz * (T)p.inv_lambda2;
+        const T gsd = (T)((comp ? p.g22 : p.g11) * dt), g12d = (T)(p.g12 * dt);
+        const T nsd = (T)(-0.5 * (comp ? p.l22 : p.l111) * dt), n12 = (T)(-0.5 * p.l12 * dt);
+        const T shift = comp ? (T)(p.detuning * dt) : (T)0;
+        static_for<0, N>([&](auto X_) {
+            constexpr int x = decltype(X_)::value;
+            const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
+            const T Vd = half_dt * fma_t(xx, xx, vyz);
+            const cpx<T> a0 = r[x];
+            const T n = cnorm<T>(a0), o = m[x];
+            const T ph = fma_t(gsd, n, fma_t(g12d, o, Vd - shift));
+            const T yl = fma_t(nsd, comp ? n : n * n, n12 * o);
+            T sn, cs;
+            sincos2_t(ph, &sn, &cs);
+            const T tl = expm1_small_t(yl);
+            const cpx<T> va = cscale<T>(cmul<T>(a0, mk<T>(cs, sn)), scale);
+            r[x] = mk<T>(fma_t(va.x, tl, va.x), fma_t(va.y, tl, va.y));
+        });
+    }
     // sin and cos of the doubled phase (IT1 path)
     static __device__ __forceinline__ void sincos2_t(T x, T *s, T *c)
     {
@@ -380,6 +412,55 @@ template <typename T, int N, bool IT1 = true, bool NOISE = false> struct OpNonli
         else sincos(x, s, c);
     }
 };
+
+// Split form of the real-time nonlinear xy pass (FP32, itmax = 1, no noise):
+// a workgroup of two wavefronts; wavefront c carries component c of the same
+// 64/N planes through the transforms.  The step couples the components only
+// through the densities, which the wavefronts exchange in the (then idle)
+// transpose tiles.  Twice the wavefronts of xy_kernel, each with half the
+// registers and work: for batches too small to fill the chip with xy_kernel.
+template <int N, bool NT>
+__global__ void __launch_bounds__(128) xy_split_kernel(OpNonlinRT<float, N, true> op, XyGeom g)
+{
+    using T = float;
+    constexpr int PPW = 64 / N;
+    __shared__ cpx<T> lds[2 * PPW * N * (N + 1)];
+    const int comp = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pl = lane / N, t = lane % N;
+    const long blk = g.xcd ? xcd_remap(blockIdx.x, g.nz / PPW) : (long)blockIdx.x;
+    const long plane = blk * PPW + pl;
+    const int z = (int)(plane % g.nz);
+    const int traj = (int)(plane / g.nz);
+    cpx<T> *tile = lds + (comp * PPW + pl) * N * (N + 1);
+    const cpx<T> *other = lds + ((1 - comp) * PPW + pl) * N * (N + 1);
+    const long base = plane * (long)(N * N);
+    cpx<T> *arr = comp ? op.b : op.a;
+
+    cpx<T> v[N];
+#pragma unroll
+    for (int y = 0; y < N; ++y) v[y] = ld<NT, T>(arr + base + y * N + t);
+    dft<T, N, -1>(v);
+    col2row<T, N>(tile, t, v);
+    dft<T, N, -1>(v);
+
+    // densities of row t, rows padded to N + 1 against bank conflicts
+    T *dm = (T *)tile + t * (N + 1);
+#pragma unroll
+    for (int x = 0; x < N; ++x) dm[x] = cnorm<T>(v[x]);
+    __syncthreads();
+    const T *dn = (const T *)other + t * (N + 1);
+    T m[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) m[x] = dn[x];
+    __syncthreads();   // the tiles are rewritten below
+    op.apply_one(v, m, comp, t, z, traj);
+
+    dft<T, N, +1>(v);
+    row2col<T, N>(tile, t, v);
+    dft<T, N, +1>(v);
+#pragma unroll
+    for (int y = 0; y < N; ++y) st<NT, T>(arr + base + y * N + t, v[y]);
+}
 
 // Read-only sample: x-space sums per trajectory
 // {sum|a|^2, sum|b|^2, Re sum conj(a) b, Im sum conj(a) b} in FP64.

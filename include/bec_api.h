@@ -331,9 +331,11 @@ int bec_time_passes(bec_ctx *ctx, int group, int iters, double *ms);
  * first z stage), one component per z workgroup (z_split), non-temporal
  * loads/stores (nt), XCD-aware workgroup placement (xcd), minimum waves per
  * SIMD the xy pass's register allocation must allow (xy_minw: 1 or 5), z LDS
- * exchange in two real-valued rounds through half the LDS (z_half). */
+ * exchange in two real-valued rounds through half the LDS (z_half), one
+ * wavefront per component in the real-time xy pass (xy_split; FP32, itmax = 1,
+ * loss noise off). */
 typedef struct bec_schedule {
-    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half;
+    int pw, z_r1, z_split, nt, xcd, xy_minw, z_half, xy_split;
     double ms_per_step;   /* filled by bec_autotune */
 } bec_schedule;
 

@@ -140,7 +140,8 @@ def test_autotune_keeps_state_and_results(bec, oracle, gs):
         outs = []
         for sch in res[:: max(1, len(res) // 6)]:
             s.set_state(a0, b0, bec.SPACE_K)
-            s.set_schedule(**{k: sch[k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half")})
+            s.set_schedule(**{k: sch[k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half",
+                                                   "xy_split")})
             s.propagate(5)
             outs.append(s.get_state(bec.SPACE_K)[0])
     for o in outs[1:]:
@@ -226,3 +227,30 @@ def test_averages_match_oracle(bec, oracle, gs, prec):
     ref = oracle.averages(oc, ax, bx)
     for k, v in ref.items():
         assert got[k] == pytest.approx(v, rel=1e-11), k
+
+
+@pytest.mark.parametrize("nxy", [16, 32])
+def test_split_xy_pass_equals_two_component_pass(bec, nxy):
+    """The split xy pass (one wavefront per component, densities exchanged in LDS)
+    performs the same operations per cell as the two-component pass."""
+    c = gpu_calc(bec, **config1_model(bec, nvx=nxy, nvy=nxy, ne=8))
+    psi = np.zeros((128, nxy, nxy), dtype=complex)
+    z, y, x = np.meshgrid(np.arange(128) - 64, np.arange(nxy) - nxy // 2, np.arange(nxy) - nxy // 2, indexing="ij")
+    psi[:] = 8.0 * np.exp(-(x ** 2 + y ** 2) / 8.0 - z ** 2 / 800.0)
+    outs = []
+    with bec.Solver(c, precision=bec.FP32) as s:
+        s.init_evolution(psi, SEED, equilibrate=False)
+        a0, b0 = s.get_state(bec.SPACE_K)
+        cands = [k for k in s.schedule_candidates() if k["xy_split"] == 1]
+        assert cands, "split candidates are offered for FP32"
+        base = {k: cands[0][k] for k in ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half")}
+        for split in (0, 1):
+            s.set_state(a0, b0, bec.SPACE_K)
+            s.set_schedule(xy_split=split, **base)
+            assert s.schedule()["xy_split"] == split
+            s.propagate(12)
+            outs.append(s.get_state(bec.SPACE_K))
+    (a_two, b_two), (a_split, b_split) = outs
+    print("split vs two-component: bitwise", np.array_equal(a_two, a_split) and np.array_equal(b_two, b_split),
+          "rel", rel_l2(a_split, a_two), rel_l2(b_split, b_two), flush=True)
+    assert rel_l2(a_split, a_two) < 1e-6 and rel_l2(b_split, b_two) < 1e-6
