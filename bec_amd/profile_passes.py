@@ -21,7 +21,8 @@ def main():
     ap.add_argument("--traj", type=int, default=2048)
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)
-    ap.add_argument("--schedule", default="", help="pw,z_r1,z_split,nt,xcd,xy_minw,z_half (default: built-in)")
+    ap.add_argument("--schedule", default="",
+                    help="pw,z_r1,z_split,nt,xcd,xy_minw,z_half[,xy_split] (default: built-in)")
     a = ap.parse_args()
     kw = dict(N=30000, nvx=16, nvy=16, nvz=128, ne=1, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
               detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
@@ -34,7 +35,7 @@ def main():
     s = B.Solver(c, precision=B.FP32)
     s.init_evolution(psi, 0x5EED, equilibrate=False)
     if a.schedule:
-        keys = ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half")
+        keys = ("pw", "z_r1", "z_split", "nt", "xcd", "xy_minw", "z_half", "xy_split")
         s.set_schedule(**dict(zip(keys, (int(v) for v in a.schedule.split(",")))))
     ms = s.time_passes(iters=a.iters)
     copy_bytes = a.copy_mib << 20
