@@ -4,7 +4,8 @@
 //
 // At these lattices (up to 64 x 64) a whole trajectory fits one workgroup's
 // LDS: both components, 64 x 65 complex doubles each (rows padded by one
-// element against bank conflicts), 133 KB of the 160 KB.  Each workgroup runs
+// element against bank conflicts), 133 KB of the 160 KB.  Each workgroup (1024
+// threads, four wavefronts per SIMD) runs
 // its trajectory's entire evolution -- noise, pulses, every split step and the
 // measurements -- with the state resident in LDS; HBM sees only the ground
 // state read at the start and the measurements written.  The 2D FFTs are
@@ -25,7 +26,8 @@ namespace bec {
 
 namespace {
 
-constexpr int THREADS = 256;
+constexpr int THREADS = 1024;   // 16 wavefronts: four per SIMD hide the LDS and FP64 latencies
+constexpr int SCRATCH = THREADS / 64 * 8;   // block_sum: up to 8 sums per wavefront
 constexpr int MAXN = 64;
 constexpr int LDMAX = MAXN + 1;
 constexpr int CPT = MAXN * MAXN / THREADS;   // cells per thread at the largest lattice
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(THREADS) void w2d_ground_state_kernel(W2dParams p, 
 {
     __shared__ double2 A[MAXN * LDMAX], S[MAXN * LDMAX];
     __shared__ double2 twx[MAXN / 2], twy[MAXN / 2];
-    __shared__ double scratch[64];
+    __shared__ double scratch[SCRATCH];
     const Geo g = make_geo(p);
     make_twiddles(g, twx, twy);
     for (int c = threadIdx.x; c < g.n; c += THREADS) A[lds_of(g, c)] = make_double2(p.n0, 0.0);
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(THREADS) void w2d_ensemble_kernel(W2dParams p, cons
 {
     __shared__ double2 A[MAXN * LDMAX], Bf[MAXN * LDMAX];
     __shared__ double2 twx[MAXN / 2], twy[MAXN / 2];
-    __shared__ double scratch[64];
+    __shared__ double scratch[SCRATCH];
     const Geo g = make_geo(p);
     const int traj = blockIdx.x;
     make_twiddles(g, twx, twy);
