@@ -73,6 +73,7 @@ struct TwPair {
 constexpr double f32_ulp(double v)   // spacing of floats around v (normal range)
 {
     double a = v < 0 ? -v : v, p = 1.0;
+    if (a < 1.1754943508222875e-38) return 1.401298464324817e-45;   // zero and subnormals
     while (p > a) p *= 0.5;
     while (p * 2.0 <= a) p *= 2.0;
     return p * 1.1920928955078125e-7;
