@@ -168,6 +168,7 @@ EXPORTS = {
     "bec_trajectory_sums": (C.c_int, [_P, C.c_int, _P]),
     "bec_projections": (C.c_int, [_P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P]),
     "bec_pulse": (C.c_int, [_P, C.c_int]),
+    "bec_pulse_rotation": (C.c_int, [_P, C.c_double, C.c_double]),
     "bec_get_state": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     "bec_set_state": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     "bec_get_time": (C.c_double, [_P]),
@@ -462,6 +463,10 @@ class Solver:
 
     def pulse(self, kind=PULSE_HALF_PI):
         check(lib().bec_pulse(self._ctx, kind), "bec_pulse")
+
+    def pulse_rotation(self, theta, phi=0.0):
+        """A resonant pulse of area theta and phase phi on every group (bec_pulse_rotation)."""
+        check(lib().bec_pulse_rotation(self._ctx, float(theta), float(phi)), "bec_pulse_rotation")
 
     def get_state(self, space=SPACE_X, group=0):
         a = np.empty(self.shape(group), dtype=np.complex128)

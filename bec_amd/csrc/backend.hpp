@@ -73,6 +73,8 @@ hipError_t run_z_fft(const DevGroup &g, const void *in, void *out, int ntraj, in
 hipError_t run_noise_init(const DevGroup &g, const void *gs, uint64_t seed, double Va, double Vb,
                           double inv_sqrt_dv, hipStream_t s);
 hipError_t run_pulse(const DevGroup &g, int kind, hipStream_t s);
+// (a, b) <- U (a, b) on every cell; u = {u00, u01, u10, u11} as (re, im) pairs
+hipError_t run_rotation(const DevGroup &g, const double u[8], hipStream_t s);
 // per-cell ensemble sums of vacuum-subtracted densities after halfPiRotate(alpha):
 // out[cell*2 + c], x-space
 hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, double *out,

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -584,6 +585,20 @@ extern "C" int bec_pulse(bec_ctx *ctx, int kind)
     CHECK_CTX(ctx);
     if (kind != BEC_PULSE_HALF_PI && kind != BEC_PULSE_PI) return BEC_ERR_INVALID;
     for (auto &g : ctx->groups) BEC_HIP(run_pulse(g.dev, kind, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+extern "C" int bec_pulse_rotation(bec_ctx *ctx, double theta, double phi)
+{
+    CHECK_CTX(ctx);
+    if (!std::isfinite(theta) || !std::isfinite(phi)) return BEC_ERR_INVALID;
+    for (auto &g : ctx->groups)
+        if (!g.initialized) return BEC_ERR_STATE;
+    const double C = cos(0.5 * theta), S = sin(0.5 * theta), cp = cos(phi), sp = sin(phi);
+    // U = [[C, -i e^{-i phi} S], [-i e^{i phi} S, C]]
+    const double u[8] = {C, 0.0, -sp * S, -cp * S, sp * S, -cp * S, C, 0.0};
+    for (auto &g : ctx->groups) BEC_HIP(run_rotation(g.dev, u, ctx->stream));
     BEC_HIP(hipStreamSynchronize(ctx->stream));
     return BEC_OK;
 }

@@ -370,6 +370,35 @@ __global__ void pulse_kernel(cpx<T> *__restrict__ a, cpx<T> *__restrict__ b, lon
     }
 }
 
+// A resonant pulse: (a, b) <- U (a, b) with U = [[u00, u01], [u10, u11]].
+template <typename T>
+__global__ void rotation_kernel(cpx<T> *__restrict__ a, cpx<T> *__restrict__ b, long n, double2 u00, double2 u01,
+                                double2 u10, double2 u11)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cpx<T> a0 = a[i], b0 = b[i];
+    const cpx<T> m00 = mk<T>((T)u00.x, (T)u00.y), m01 = mk<T>((T)u01.x, (T)u01.y);
+    const cpx<T> m10 = mk<T>((T)u10.x, (T)u10.y), m11 = mk<T>((T)u11.x, (T)u11.y);
+    a[i] = cadd<T>(cmul<T>(m00, a0), cmul<T>(m01, b0));
+    b[i] = cadd<T>(cmul<T>(m10, a0), cmul<T>(m11, b0));
+}
+
+hipError_t run_rotation(const DevGroup &g, const double u[8], hipStream_t s)
+{
+    long n = (long)g.nx * g.ny * g.nz * g.ntraj;
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const double2 u00 = make_double2(u[0], u[1]), u01 = make_double2(u[2], u[3]);
+    const double2 u10 = make_double2(u[4], u[5]), u11 = make_double2(u[6], u[7]);
+    if (g.prec == 0)
+        hipLaunchKernelGGL(rotation_kernel<float>, grid, block, 0, s, (float2 *)g.a, (float2 *)g.b, n, u00, u01, u10,
+                           u11);
+    else
+        hipLaunchKernelGGL(rotation_kernel<double>, grid, block, 0, s, (double2 *)g.a, (double2 *)g.b, n, u00, u01,
+                           u10, u11);
+    return hipGetLastError();
+}
+
 hipError_t run_pulse(const DevGroup &g, int kind, hipStream_t s)
 {
     long n = (long)g.nx * g.ny * g.nz * g.ntraj;

@@ -313,6 +313,15 @@ int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy_a, double 
 /* applyHalfPiPulse / applyPiPulse (wigner_kernel.cu:320-341) on every group. */
 int bec_pulse(bec_ctx *ctx, int kind);
 
+/* A resonant pulse of area theta and phase phi on every group:
+ *   (a, b) <- U (a, b),  U = [[cos(theta/2), -i e^{-i phi} sin(theta/2)],
+ *                             [-i e^{+i phi} sin(theta/2), cos(theta/2)]].
+ * U(pi/2, 0) and U(pi, 0) are applyHalfPiPulse and applyPiPulse
+ * (wigner_kernel.cu:320-341); U(pi/2, -alpha) is the measurement rotation
+ * halfPiRotate(alpha) (wigner_kernel.cu:344-360).  The pulse is local, so it
+ * acts on the stored state in any representation. */
+int bec_pulse_rotation(bec_ctx *ctx, double theta, double phi);
+
 /* Checkpoint: read / write the state of a group in x- or k-space
  * (k-space: psi_k = (1/cells) sum_x psi_x exp(+i k.x), the reference's). */
 int bec_get_state(bec_ctx *ctx, int group, int space, double *a, double *b);

@@ -1340,3 +1340,17 @@ void orc_averages(const orc_calc *c, const orc_c *a, const orc_c *b, int ne, dou
     out[1] = norm;
     for (int k = 0; k < 4; k++) out[2 + k] = v[k] / ((double)c->cells * ne) * norm;
 }
+
+/* A resonant pulse of area theta and phase phi:
+ * U = [[cos(theta/2), -i e^{-i phi} sin(theta/2)], [-i e^{i phi} sin(theta/2), cos(theta/2)]]
+ * (U(pi/2, 0), U(pi, 0): wigner_kernel.cu:320-341; U(pi/2, -alpha): halfPiRotate, :344-360). */
+void orc_rotation(orc_c *a, orc_c *b, long n, double theta, double phi)
+{
+    double C = cos(0.5 * theta), S = sin(0.5 * theta);
+    orc_c u01 = cmk(-sin(phi) * S, -cos(phi) * S), u10 = cmk(sin(phi) * S, -cos(phi) * S);
+    for (long i = 0; i < n; i++) {
+        orc_c a0 = a[i], b0 = b[i];
+        a[i] = cadd(cscale(a0, C), cmul(u01, b0));
+        b[i] = cadd(cmul(u10, a0), cscale(b0, C));
+    }
+}

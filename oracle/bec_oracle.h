@@ -179,6 +179,9 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
               double *avn, orc2d_report *rep);
 double orc2d_residual(const orc2d_params *p, const orc_c *psi_x, double *mu_out);
 
+/* resonant pulse of area theta and phase phi on n cells (U(pi/2,0), U(pi,0) = the reference pulses) */
+void orc_rotation(orc_c *a, orc_c *b, long n, double theta, double phi);
+
 /* calculateAverages (wigner.cu:305-337) on x-space states: {N, norm, var Re a, var Im a, var Re b, var Im b} */
 void orc_averages(const orc_calc *c, const orc_c *a_x, const orc_c *b_x, int ne, double out[6]);
 

@@ -124,6 +124,7 @@ def lib():
             "orc_to_xspace": (None, [C.POINTER(Calc), _P, C.c_int]),
             "orc_init_evolution": (None, [C.POINTER(Calc), _P, C.c_uint64, C.c_int, _P, _P]),
             "orc_propagate": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_int]),
+            "orc_rotation": (None, [_P, _P, C.c_long, C.c_double, C.c_double]),
             "orc_averages": (None, [C.POINTER(Calc), _P, _P, C.c_int, _P]),
             "orc_loss_noise_cell": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
             "orc_nonlinear_rt_noise": (None, [C.POINTER(Calc), _P, _P, C.c_int, C.c_double, C.c_uint64,
@@ -354,6 +355,13 @@ def propagate(c, a_k, b_k, dt, n_steps):
     b = _c128(b_k).copy()
     ne = a.size // c.cells
     lib().orc_propagate(C.byref(c), _ptr(a), _ptr(b), ne, dt, n_steps)
+    return a, b
+
+
+def rotation(a, b, theta, phi=0.0):
+    """A resonant pulse of area theta and phase phi (orc_rotation)."""
+    a, b = _c128(a).copy(), _c128(b).copy()
+    lib().orc_rotation(_ptr(a), _ptr(b), a.size, theta, phi)
     return a, b
 
 

@@ -122,6 +122,7 @@ def test_invalid_arguments_rejected_on_cpu():
     assert L.bec_run_1d(C.byref(bec_amd.params1d()), None, v1, None, None, None, None, None) == api.ERR_INVALID
     # context-less calls
     assert L.bec_set_loss_noise(None, 1, 5, 0) == api.ERR_INVALID
+    assert L.bec_pulse_rotation(None, 1.0, 0.0) == api.ERR_INVALID
     step = C.c_uint(0)
     assert L.bec_get_loss_noise_step(None, 0, C.byref(step)) == api.ERR_INVALID
     assert L.bec_propagate(None, 1, 0.0, 0, None) == api.ERR_INVALID

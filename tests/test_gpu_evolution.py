@@ -254,3 +254,24 @@ def test_split_xy_pass_equals_two_component_pass(bec, nxy):
     print("split vs two-component: bitwise", np.array_equal(a_two, a_split) and np.array_equal(b_two, b_split),
           "rel", rel_l2(a_split, a_two), rel_l2(b_split, b_two), flush=True)
     assert rel_l2(a_split, a_two) < 1e-6 and rel_l2(b_split, b_two) < 1e-6
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-14), (0, 1e-6)])
+def test_pulse_rotation_matches_oracle(bec, oracle, gs, prec, tol):
+    """bec_pulse_rotation(theta, phi) against orc_rotation on the same state; at
+    (pi/2, 0) it is the reference's half-pi pulse."""
+    c, _ = _pair(bec, oracle, prec, ne=2)
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(gs, SEED, equilibrate=False)
+        a0, b0 = s.get_state(bec.SPACE_K)
+        s.pulse_rotation(1.3, -0.8)
+        a1, b1 = s.get_state(bec.SPACE_K)
+        s.set_state(a0, b0, bec.SPACE_K)
+        s.pulse_rotation(np.pi / 2, 0.0)
+        a2, b2 = s.get_state(bec.SPACE_K)
+        s.set_state(a0, b0, bec.SPACE_K)
+        s.pulse(bec.PULSE_HALF_PI)
+        a3, b3 = s.get_state(bec.SPACE_K)
+    ra, rb = oracle.rotation(a0, b0, 1.3, -0.8)
+    assert rel_l2(a1, ra) < tol and rel_l2(b1, rb) < tol
+    assert rel_l2(a2, a3) < tol and rel_l2(b2, b3) < tol
