@@ -251,8 +251,9 @@ def main():
     ap.add_argument("--uneven-steps", type=int, default=400)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); trajectories are independent, "
-              "see the sharding note in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): trajectories are independent and "
+              "would shard across GPUs with one reduction of the FP64 sample sums, which this build does not run",
+              file=sys.stderr)
         sys.exit(2)
 
     import bec_amd as B
