@@ -291,6 +291,9 @@ template <typename T, int N, bool IT1 = true, bool NOISE = false> struct OpNonli
             });
             return;
         }
+        // itmax = 0 never forms the factor: the output is zero, as the reference's
+        // uninitialised factor (quirk Q10) and the oracle give
+        const T scale_out = itmax > 0 ? scale : (T)0;
         static_for<0, N>([&](auto X_) {
             constexpr int x = decltype(X_)::value;
             const T xx = (T)(-p.xmax) + (T)p.dx * (T)x;
@@ -320,8 +323,8 @@ template <typename T, int N, bool IT1 = true, bool NOISE = false> struct OpNonli
                 factors(cscale<T>(cmul<T>(a0, ea), sqrt_t((T)1 + ta)),
                         cscale<T>(cmul<T>(b0, eb), sqrt_t((T)1 + tb)));
             }
-            const cpx<T> va = cscale<T>(cmul<T>(a0, cmul<T>(ea, ea)), scale);
-            const cpx<T> vb = cscale<T>(cmul<T>(b0, cmul<T>(eb, eb)), scale);
+            const cpx<T> va = cscale<T>(cmul<T>(a0, cmul<T>(ea, ea)), scale_out);
+            const cpx<T> vb = cscale<T>(cmul<T>(b0, cmul<T>(eb, eb)), scale_out);
             r[0][x] = mk<T>(fma_t(va.x, ta, va.x), fma_t(va.y, ta, va.y));
             r[1][x] = mk<T>(fma_t(vb.x, tb, vb.x), fma_t(vb.y, tb, vb.y));
             add_noise(x, a0, b0);

@@ -127,6 +127,22 @@ def test_midpoint_iterations_match_oracle(bec, oracle, gs, prec, tol):
     assert rel_l2(bx, oracle.to_xspace(oc, ob)) < tol
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_zero_midpoint_iterations_give_zero_state(bec, oracle, gs, prec):
+    """itmax = 0 never forms the nonlinear factor: the reference's kernel then
+    multiplies by an uninitialised factor (quirk Q10); the oracle and the GPU
+    both give zero."""
+    c, oc = _pair(bec, oracle, prec, ne=2, itmax=0)
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(gs, SEED, equilibrate=False)
+        s.propagate(3)
+        ax, bx = s.get_state(bec.SPACE_X)
+    oa, ob = oracle.init_evolution(oc, gs, SEED, equilibrate=False)
+    oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, 3)
+    assert not np.any(oa) and not np.any(ob)
+    assert not np.any(ax) and not np.any(bx)
+
+
 def test_autotune_keeps_state_and_results(bec, oracle, gs):
     """The autotuner restores the state; every schedule computes the same step."""
     c, _ = _pair(bec, oracle, 0, ne=8)
