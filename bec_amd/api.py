@@ -18,6 +18,7 @@ SPACE_X, SPACE_K = 0, 1
 PULSE_HALF_PI, PULSE_PI = 0, 1
 FFT_FORWARD, FFT_INVERSE = -1, 1
 GS_SPLIT_STEP, GS_GRADIENT = 0, 1
+GS2_DEFAULT_TAU, GS2_DEFAULT_BETA = 1.0, 0.85   # the coupled flow's step and momentum (bec_api.h)
 GS_RULE_REFERENCE, GS_RULE_RESIDUAL = 0, 1
 
 
@@ -288,6 +289,8 @@ def ground_states_two_component(params, frac_a=None, guesses=None, **opt):
     arr = (CalcParams * B)(*params)
     opt.setdefault("method", GS_GRADIENT)
     opt.setdefault("rule", GS_RULE_RESIDUAL)
+    opt.setdefault("tau", GS2_DEFAULT_TAU)
+    opt.setdefault("beta", GS2_DEFAULT_BETA)
     o = gs_options(**opt)
     p0 = params[0]
     out = np.empty((B, 2, p0.nvz, p0.nvy, p0.nvx), dtype=np.complex128)

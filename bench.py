@@ -211,8 +211,17 @@ def surrogate_summary():
         return None
     with open(SURROGATE_SUMMARY) as f:
         d = json.load(f)
-    return {"source": os.path.relpath(SURROGATE_SUMMARY, ROOT), "command": d.get("command"),
-            "summary": d.get("summary")}
+    keep = ("held_out", "tf_iterations_mean", "nn_iterations_mean", "tf_ms_mean", "nn_ms_mean_incl_inference",
+            "speedup_mean_time", "speedup_iterations")
+    cut = lambda s: {k: s[k] for k in keep if k in s}
+    out = {"source": os.path.relpath(SURROGATE_SUMMARY, ROOT), "command": d.get("command"),
+           "same_flow_both_starts": cut(d["summary"])}
+    for spec, s in (d.get("nn_sweep") or {}).items():
+        out[f"network_start_flow_{spec}"] = cut(s)
+    for k, s in d.items():
+        if k.startswith("both_starts_"):
+            out[k] = cut(s)
+    return out
 
 
 def cpu_baseline(psi, seconds):

@@ -133,6 +133,11 @@ typedef struct bec_gs_report {
 
 int bec_default_gs_options(bec_gs_options *out);
 
+/* Step and momentum of the two-component gradient flow (the one-component
+ * defaults of bec_default_gs_options are 0.5 and 0.6). */
+#define BEC_GS2_DEFAULT_TAU 1.0
+#define BEC_GS2_DEFAULT_BETA 0.85
+
 /* calculateSteadyState (wigner.cu:84-175) for group `group`: imaginary-time
  * ground state of component |1>, returned in x-space (cells complex values).
  * guess_x (x-space, may be NULL) replaces the Thomas-Fermi start (K1). */
@@ -153,7 +158,8 @@ int bec_ground_states_batched(const bec_calc_params *params, int batch, const do
  *   mu_a a = (K + V + g11|a|^2 + g12|b|^2) a,  mu_b b = (K + V + g22|b|^2 + g12|a|^2) b
  * with dV sum|a|^2 = frac_a N and dV sum|b|^2 = (1 - frac_a) N, solved by the
  * preconditioned gradient flow (method BEC_GS_GRADIENT, rule
- * BEC_GS_RULE_RESIDUAL; opt NULL selects them).  frac_a may be NULL (1/2 each).
+ * BEC_GS_RULE_RESIDUAL; opt NULL selects them, with the coupled flow's own
+ * step and momentum BEC_GS2_DEFAULT_TAU / _BETA).  frac_a may be NULL (1/2 each).
  * guesses and out_x hold batch * 2 * cells complex values, x-space, laid out
  * [set][component][cell]; guesses NULL gives the Thomas-Fermi start (the
  * one-component TF shape of N atoms, scaled to each component's number). */

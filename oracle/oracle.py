@@ -289,8 +289,9 @@ def tf_state2(c, frac_a=0.5):
     return a, b
 
 
-def ground_state2(c, frac_a=0.5, guess=None, tol=1e-8, max_iter=5000, alpha=None, tau=0.5, beta=0.6):
-    """Two-component ground state (gradient flow); returns (a, b, report)."""
+def ground_state2(c, frac_a=0.5, guess=None, tol=1e-8, max_iter=5000, alpha=None, tau=1.0, beta=0.85):
+    """Two-component ground state (gradient flow, with the library's default step and
+    momentum of the coupled flow); returns (a, b, report)."""
     a = np.empty(shape(c), dtype=np.complex128)
     b = np.empty(shape(c), dtype=np.complex128)
     rep = GS2Report()

@@ -4,7 +4,10 @@ A reduced network (fewer modes and channels than the production one, same
 code path) is trained by the library's own kernels on ground states that the
 library's solver generates; on held-out parameter sets the solve started from
 its guess must reach the same residual as the Thomas-Fermi-started solve in
-fewer iterations."""
+fewer iterations, both with the library's default flow (tau 1.0, beta 0.85)
+and with the slower original one (tau 0.5, beta 0.6).  The reduced network
+measured 143 against 175 iterations with the first and 470 to 489 against 782
+with the second; the bars are 0.9 and 0.8 of the Thomas-Fermi count."""
 import numpy as np
 import pytest
 
@@ -22,12 +25,13 @@ def test_trained_network_cuts_iterations(bec):
     hist, secs = S.train(net, X[:64], Y[:64], epochs=40, batch=8, lr=3e-3, log=lambda m: None)
     print("training loss", hist[0], "->", hist[-1], f"({secs:.1f} s)")
     assert hist[-1] < 0.1 * hist[0]
-    rows = S.evaluate(net, thetas[64:], X[64:], Y[64:], log=print)
-    s = S.summary(rows)
-    print("held-out:", s)
-    assert s["converged_all"]
-    assert s["final_residual_max"]["nn"] < 1e-6 and s["final_residual_max"]["tf"] < 1e-6
-    assert s["nn_iterations_mean"] < 0.8 * s["tf_iterations_mean"]
+    for flow, bar in ((None, 0.9), (dict(tau=0.5, beta=0.6), 0.8)):
+        rows = S.evaluate(net, thetas[64:], X[64:], Y[64:], log=print, tf_opts=flow, nn_opts=flow)
+        s = S.summary(rows)
+        print("held-out, flow", flow or "default", ":", s)
+        assert s["converged_all"]
+        assert s["final_residual_max"]["nn"] < 1e-6 and s["final_residual_max"]["tf"] < 1e-6
+        assert s["nn_iterations_mean"] < bar * s["tf_iterations_mean"]
 
 
 def test_features_and_targets_layout(bec):
