@@ -63,19 +63,21 @@ def calc_for(theta, nz=128, nxy=16):
 
 
 def features(c, theta):
-    """[cells][16] float32 inputs for one parameter set (cell = z ny nx + y nx + x)."""
-    x = -c.xmax + c.dx * np.arange(c.nvx)
-    y = -c.ymax + c.dy * np.arange(c.nvy)
-    z = -c.zmax + c.dz * np.arange(c.nvz)
-    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
-    V = 0.5 * (X ** 2 + Y ** 2 + Z ** 2 / c.lambda_ ** 2)
-    f = np.zeros((c.nvz, c.nvy, c.nvx, C_IN), dtype=np.float32)
+    """[cells][16] float32 inputs for one parameter set (cell = z ny nx + y nx + x).
+    The coordinates stay separable (broadcast, not meshed) until the potential
+    is formed; the values are those of the full-grid formulas."""
+    x = (-c.xmax + c.dx * np.arange(c.nvx))[None, None, :]
+    y = (-c.ymax + c.dy * np.arange(c.nvy))[None, :, None]
+    z = (-c.zmax + c.dz * np.arange(c.nvz))[:, None, None]
+    V = 0.5 * (x ** 2 + y ** 2 + z ** 2 / c.lambda_ ** 2)
+    f = np.empty((c.nvz, c.nvy, c.nvx, C_IN), dtype=np.float32)
     f[..., 0] = np.sqrt(np.maximum(c.mu - V, 0.0) / c.mu)
     f[..., 1] = V / c.mu
-    f[..., 2] = X / c.xmax
-    f[..., 3] = Y / c.ymax
-    f[..., 4] = Z / c.zmax
+    f[..., 2] = x / c.xmax
+    f[..., 3] = y / c.ymax
+    f[..., 4] = z / c.zmax
     f[..., 5:9] = scaled(theta)
+    f[..., 9:] = 0.0
     return f.reshape(-1, C_IN)
 
 
@@ -112,7 +114,10 @@ def guesses(c, pred):
     """Network output [cells][2] -> solver guess [2][cells] complex."""
     u = amp_unit(c)
     a, b = from_net(np.asarray(pred, dtype=np.float64))
-    return (np.stack([a, b]) * u).astype(np.complex128)
+    g = np.zeros((2, a.shape[0]), dtype=np.complex128)
+    g.real[0] = a * u
+    g.real[1] = b * u
+    return g
 
 
 def guess_error(pred, target):
@@ -201,6 +206,14 @@ def rel_l2(a, b):
 NN_FLOW = "1.0:0.7"
 
 
+def flow(spec):
+    """'tau:beta' -> solve() options ({} for an empty spec: the library's defaults)."""
+    if not spec:
+        return {}
+    tau, beta = (float(v) for v in spec.split(":"))
+    return dict(tau=tau, beta=beta)
+
+
 def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None):
     """Per held-out set: solve from TF and from the network guess (one set per solve).
     inference_ms is the host wall clock of the whole guess: input features,
@@ -268,12 +281,6 @@ def main(argv=None):
     ap.add_argument("--load-net", default=None, help="skip training and load the parameters from this .npy")
     a = ap.parse_args(argv)
 
-    def opts(spec):
-        if not spec:
-            return {}
-        tau, beta = (float(v) for v in spec.split(":"))
-        return dict(tau=tau, beta=beta)
-
     def log(msg):
         print(msg, flush=True)
 
@@ -291,7 +298,7 @@ def main(argv=None):
                               log=log, time_limit=a.time_limit)
     if a.save_net:
         np.save(a.save_net, np.asarray(net.get_params(), dtype=np.float32))
-    tf_opts, nn_opts = opts(a.tf), opts(a.nn)
+    tf_opts, nn_opts = flow(a.tf), flow(a.nn)
     rows = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log, tf_opts=tf_opts, nn_opts=nn_opts)
     res = {"config": dict(NET, train=a.train, test=a.test, epochs_run=len(hist), batch=a.batch, lr=a.lr,
                           seed=a.seed, tf_flow=tf_opts or "default", nn_flow=nn_opts or "default"),
@@ -299,13 +306,13 @@ def main(argv=None):
            "train_loss": hist, "summary": summary(rows)}
     sweep = {}
     for spec in [v for v in a.nn_sweep.split(",") if v]:
-        r2 = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log, tf_opts=tf_opts, nn_opts=opts(spec))
+        r2 = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log, tf_opts=tf_opts, nn_opts=flow(spec))
         sweep[spec] = summary(r2)
         log(f"network-started flow {spec}: {json.dumps(sweep[spec])}")
     if sweep:
         res["nn_sweep"] = sweep
     if a.legacy:
-        lo = opts(a.legacy)
+        lo = flow(a.legacy)
         r3 = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log, tf_opts=lo, nn_opts=lo)
         res["both_starts_" + a.legacy] = summary(r3)
         log(f"both starts with {a.legacy}: {json.dumps(res['both_starts_' + a.legacy])}")
