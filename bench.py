@@ -105,7 +105,8 @@ def uneven_groups(B, psi_a, steps, autotune=True):
             "ground_state_256x32x32": {"iterations": rep_b["iterations"], "residual": rep_b["residual"]}}
 
 
-SURROGATE_SUMMARY = os.path.join(ROOT, "profiles", "surrogate_groundstate_200heldout.json")
+SURROGATE_SUMMARIES = [os.path.join(ROOT, "profiles", n) for n in
+                       ("surrogate_groundstate_200heldout.json", "surrogate_flow_sweep_200heldout.json")]
 
 
 def program_2d(B, with_cpu):
@@ -206,22 +207,29 @@ def two_component_ground_state(B):
 
 
 def surrogate_summary():
-    """The committed result of `python -m bec_amd.surrogate` (not measured by this run)."""
-    if not os.path.exists(SURROGATE_SUMMARY):
-        return None
-    with open(SURROGATE_SUMMARY) as f:
-        d = json.load(f)
-    keep = ("held_out", "tf_iterations_mean", "nn_iterations_mean", "tf_ms_mean", "nn_ms_mean_incl_inference",
-            "speedup_mean_time", "speedup_iterations")
+    """The committed results of `python -m bec_amd.surrogate` (not measured by this run): the
+    Thomas-Fermi-started solve against the guess-started one, for each flow setting reported."""
+    keep = ("held_out", "converged_all", "tf_iterations_mean", "nn_iterations_mean", "tf_ms_mean",
+            "nn_ms_mean_incl_inference", "speedup_mean_time", "speedup_iterations")
     cut = lambda s: {k: s[k] for k in keep if k in s}
-    out = {"source": os.path.relpath(SURROGATE_SUMMARY, ROOT), "command": d.get("command"),
-           "same_flow_both_starts": cut(d["summary"])}
-    for spec, s in (d.get("nn_sweep") or {}).items():
-        out[f"network_start_flow_{spec}"] = cut(s)
-    for k, s in d.items():
-        if k.startswith("both_starts_"):
-            out[k] = cut(s)
-    return out
+    out = {"sources": [], "commands": []}
+    for path in SURROGATE_SUMMARIES:
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            d = json.load(f)
+        out["sources"].append(os.path.relpath(path, ROOT))
+        out["commands"].append(d.get("command"))
+        nn = d["config"].get("nn_flow")
+        key = ("same_flow_both_starts" if nn == "default"
+               else f"network_start_flow_{nn['tau']}:{nn['beta']}")
+        out.setdefault(key, cut(d["summary"]))
+        for spec, s in (d.get("nn_sweep") or {}).items():
+            out.setdefault(f"network_start_flow_{spec}", cut(s))
+        for k, s in d.items():
+            if k.startswith("both_starts_"):
+                out.setdefault(k, cut(s))
+    return out if out["sources"] else None
 
 
 def cpu_baseline(psi, seconds):
