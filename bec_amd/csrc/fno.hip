@@ -104,6 +104,35 @@ __global__ void mse_grad(const float *y, const float *t, float *dy, long n, floa
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+__device__ __forceinline__ long map_off(const bec_map2 &m, long i)
+{
+    return (i >> m.lg) * m.hi + (i & ((1L << m.lg) - 1)) * m.lo;
+}
+
+// The per-mode channel mixing for a single sample (one row per mode):
+// C[t][n] = sum_k bf16(A[t][k]) E[t][k][n], one wave per mode t, lane n < N,
+// k in order.  The operands are rounded as the GEMM rounds them (A to BF16
+// nearest even, E already BF16); the products are exact in FP32.
+__global__ __launch_bounds__(256) void mode_row_mix(const float *A, const unsigned short *E, float *C, bec_map2 ab,
+                                                    bec_map2 ak, bec_map2 bb, bec_map2 bk, bec_map2 bn, bec_map2 cb,
+                                                    bec_map2 cn, int K, int N, long modes)
+{
+    const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int n = threadIdx.x & 63;
+    if (t >= modes || n >= N) return;
+    const float *a = A + map_off(ab, t);
+    const unsigned short *e = E + map_off(bb, t) + map_off(bn, n);
+    float acc = 0.f;
+#pragma unroll 8
+    for (int k = 0; k < K; ++k) {
+        const float x = a[map_off(ak, k)];
+        const unsigned int u = __float_as_uint(x);
+        const float xr = (u & 0x7f800000u) == 0x7f800000u ? x : __uint_as_float((unsigned int)to_bf16(x) << 16);
+        acc = fmaf(xr, __uint_as_float((unsigned int)e[map_off(bk, k)] << 16), acc);
+    }
+    C[map_off(cb, t) + map_off(cn, n)] = acc;
+}
+
 // E[t] (2W x 2W, BF16) from R[t][i][o][re, im]
 __global__ void expand_modes(const float *R, unsigned short *E, int W, long modes)
 {
@@ -383,6 +412,25 @@ ModeMaps mode_maps(const Spec &q)
     return m;
 }
 
+// The channel mixing of a batch-1 forward pass (N = K = 2W <= 64) on
+// mode_row_mix instead of 64 x 64 GEMM tiles that would hold one row each;
+// the operand extents are checked as G checks them.
+hipError_t mode_mix_row(const bec_fno *f, const float *A, const unsigned short *El, float *C, int W2,
+                        const ModeMaps &mm, long modes, hipStream_t s)
+{
+    const bec_map2 bb = m1((long)W2 * W2), bk = m1(W2), bn = m1(1);
+    const long ea = max_off(mm.batch, modes) + max_off(mm.ch, W2);
+    const long eb = max_off(bb, modes) + max_off(bk, W2) + max_off(bn, W2);
+    const long ec = max_off(mm.batch, modes) + max_off(mm.ch, W2);
+    if (W2 > 64 || !(ea < room(f, A, 4) && eb < room(f, El, 2) && ec < room(f, C, 4))) {
+        set_last_error("surrogate mode mixing exceeds its buffer");
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(mode_row_mix, dim3((unsigned)((modes + 3) / 4)), dim3(256), 0, s, A, El, C, mm.batch, mm.ch,
+                       bb, bk, bn, mm.batch, mm.ch, W2, W2, modes);
+    return hipGetLastError();
+}
+
 hipError_t forward_impl(bec_fno *f, int B, const float *x, hipStream_t s)
 {
     const bec_fno_config &c = f->c;
@@ -401,8 +449,11 @@ hipError_t forward_impl(bec_fno *f, int B, const float *x, hipStream_t s)
         if ((e = zdft(f, q, hin, f->zf, 0, 0.0f, s)) != hipSuccess) return e;
         if ((e = xydft(f, q, f->zf, f->sp[l], 0, s)) != hipSuccess) return e;
         const unsigned short *El = f->E + (long)l * modes * 4 * W * W;
-        if ((e = G(f, f->sp[l], 0, El, 1, f->tm, B, 2 * W, 2 * W, mm.row, mm.ch, m1(2 * W), m1(1), mm.row, mm.ch,
-                   (int)modes, mm.batch, m1(4L * W * W), mm.batch, 0.0f, nullptr, 1, nullptr, s)) != hipSuccess)
+        if (B == 1 && 2 * W <= 64) {
+            if ((e = mode_mix_row(f, f->sp[l], El, f->tm, 2 * W, mm, modes, s)) != hipSuccess) return e;
+        } else if ((e = G(f, f->sp[l], 0, El, 1, f->tm, B, 2 * W, 2 * W, mm.row, mm.ch, m1(2 * W), m1(1), mm.row,
+                          mm.ch, (int)modes, mm.batch, m1(4L * W * W), mm.batch, 0.0f, nullptr, 1, nullptr, s)) !=
+                   hipSuccess)
             return e;
         if ((e = xydft(f, q, f->tm, f->zi, 1, s)) != hipSuccess) return e;
         if ((e = zdft(f, q, f->zi, ul, 1, 0.0f, s)) != hipSuccess) return e;

@@ -88,3 +88,23 @@ def test_adam_reduces_the_loss(bec):
         net.adam_step(3e-3)
     print("loss", losses[0], "->", losses[-1])
     assert losses[-1] < 0.2 * losses[0]
+
+
+def test_production_size_forward_matches_oracle(bec):
+    """The production network at batch 1 (the per-mode channel mixing on its one-row
+    kernel, the plane transforms split in K) and at batch 2 (the mixing on the GEMM):
+    the forward pass equals the FP64 oracle to the BF16 bar."""
+    from bec_amd import fno as F
+    from bec_amd import surrogate as S
+    from oracle import fno_oracle as FO
+    net = F.FNO(seed=5, max_batch=16, **S.NET)
+    ocfg = FO.Config.from_struct(net.cfg)
+    flat = net.get_params().astype(np.float64)
+    th = S.sample_params(2, seed=9)
+    x = np.stack([S.features(S.calc_for(t), t) for t in th])
+    for b in (1, 2):
+        y = net.predict(x[:b])
+        ref = FO.forward(ocfg, flat, _grid(x[:b], ocfg)).reshape(y.shape)
+        err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        print(f"batch {b}: forward rel L2 vs FP64 oracle {err:.2e}")
+        assert np.abs(ref).max() > 0 and err < 1e-2
