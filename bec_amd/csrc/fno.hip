@@ -109,28 +109,48 @@ __device__ __forceinline__ long map_off(const bec_map2 &m, long i)
     return (i >> m.lg) * m.hi + (i & ((1L << m.lg) - 1)) * m.lo;
 }
 
-// The per-mode channel mixing for a single sample (one row per mode):
-// C[t][n] = sum_k bf16(A[t][k]) E[t][k][n], one wave per mode t, lane n < N,
-// k in order.  The operands are rounded as the GEMM rounds them (A to BF16
-// nearest even, E already BF16); the products are exact in FP32.
-__global__ __launch_bounds__(256) void mode_row_mix(const float *A, const unsigned short *E, float *C, bec_map2 ab,
-                                                    bec_map2 ak, bec_map2 bb, bec_map2 bk, bec_map2 bn, bec_map2 cb,
-                                                    bec_map2 cn, int K, int N, long modes)
+// The per-mode channel mixing for a single sample (one row per mode) with
+// 2W = 64 channels: C[t][n] = sum_k bf16(A[t][k]) E[t][k][n].  One wave per
+// mode t; lane L owns the channels n = 8 (L & 7) + i (i < 8) and the rows
+// k = (L >> 3) + 8 j, and reads E[t] in 16-byte pieces, so one load of the
+// wave covers 8 whole rows.  The 8 partial sums of a channel are combined
+// across lanes in a fixed butterfly order.  The operands are rounded as the
+// GEMM rounds them (A to BF16 nearest even, E already BF16); the products are
+// exact in FP32.
+__global__ __launch_bounds__(256) void mode_row_mix64(const float *A, const unsigned short *E, float *C, bec_map2 ab,
+                                                      bec_map2 ak, long ebatch, bec_map2 cb, bec_map2 cn, int K,
+                                                      long modes)
 {
     const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int n = threadIdx.x & 63;
-    if (t >= modes || n >= N) return;
+    if (t >= modes) return;   // t is the same for the whole wave
+    const int lane = threadIdx.x & 63, nb = lane & 7, kp = lane >> 3;
     const float *a = A + map_off(ab, t);
-    const unsigned short *e = E + map_off(bb, t) + map_off(bn, n);
-    float acc = 0.f;
-#pragma unroll 8
-    for (int k = 0; k < K; ++k) {
+    const unsigned short *e = E + t * ebatch + nb * 8;
+    float acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0.f;
+#pragma unroll 4
+    for (int k = kp; k < K; k += 8) {
         const float x = a[map_off(ak, k)];
         const unsigned int u = __float_as_uint(x);
         const float xr = (u & 0x7f800000u) == 0x7f800000u ? x : __uint_as_float((unsigned int)to_bf16(x) << 16);
-        acc = fmaf(xr, __uint_as_float((unsigned int)e[map_off(bk, k)] << 16), acc);
+        const uint4 w = *(const uint4 *)(e + (long)k * 64);
+        const unsigned int ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[2 * i] = fmaf(xr, __uint_as_float(ws[i] << 16), acc[2 * i]);
+            acc[2 * i + 1] = fmaf(xr, __uint_as_float(ws[i] & 0xffff0000u), acc[2 * i + 1]);
+        }
     }
-    C[map_off(cb, t) + map_off(cn, n)] = acc;
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i] += __shfl_xor(acc[i], off);
+    if (kp == 0) {
+        const long cbase = map_off(cb, t);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) C[cbase + map_off(cn, nb * 8 + i)] = acc[i];
+    }
 }
 
 // E[t] (2W x 2W, BF16) from R[t][i][o][re, im]
@@ -306,10 +326,11 @@ hipError_t G(const bec_fno *f, const void *A, int abf, const void *B, int bbf, f
     const long ec = max_off(cb, batch) + max_off(cm, M) + max_off(cn, N);
     // A GEMM that fills few workgroups (small batches: the transforms of a
     // single-sample forward pass) walks its K tiles in one dependent chain
-    // per workgroup; split K so the chip fills, reduced in a fixed order.
+    // per workgroup; split K to about 1024 workgroups (4 per CU; 512 left the
+    // single-sample pass at 0.57 ms against 0.49), reduced in a fixed order.
     const long wgs = ((long)(N + 63) / 64) * ((M + 63) / 64) * batch;
     if (ksplit == 1 && wgs < 256 && K >= 128) {
-        ksplit = (int)std::min<long>(std::min<long>(8, (512 + wgs - 1) / wgs), K / 64);
+        ksplit = (int)std::min<long>(std::min<long>(8, (1024 + wgs - 1) / wgs), K / 64);
         work = f->work;
     }
     bool ok = ea < room(f, A, abf ? 2 : 4) && eb < room(f, B, bbf ? 2 : 4) && ec < room(f, C, 4);
@@ -412,22 +433,22 @@ ModeMaps mode_maps(const Spec &q)
     return m;
 }
 
-// The channel mixing of a batch-1 forward pass (N = K = 2W <= 64) on
-// mode_row_mix instead of 64 x 64 GEMM tiles that would hold one row each;
-// the operand extents are checked as G checks them.
+// The channel mixing of a batch-1 forward pass with 2W = 64 on mode_row_mix64
+// instead of 64 x 64 GEMM tiles that would hold one row each; the operand
+// extents are checked as G checks them.
 hipError_t mode_mix_row(const bec_fno *f, const float *A, const unsigned short *El, float *C, int W2,
                         const ModeMaps &mm, long modes, hipStream_t s)
 {
-    const bec_map2 bb = m1((long)W2 * W2), bk = m1(W2), bn = m1(1);
+    const long eb = (modes - 1) * (long)W2 * W2 + (long)W2 * W2 - 1;
     const long ea = max_off(mm.batch, modes) + max_off(mm.ch, W2);
-    const long eb = max_off(bb, modes) + max_off(bk, W2) + max_off(bn, W2);
     const long ec = max_off(mm.batch, modes) + max_off(mm.ch, W2);
-    if (W2 > 64 || !(ea < room(f, A, 4) && eb < room(f, El, 2) && ec < room(f, C, 4))) {
+    if (W2 != 64 || ((uintptr_t)El & 15) != 0 ||
+        !(ea < room(f, A, 4) && eb < room(f, El, 2) && ec < room(f, C, 4))) {
         set_last_error("surrogate mode mixing exceeds its buffer");
         return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(mode_row_mix, dim3((unsigned)((modes + 3) / 4)), dim3(256), 0, s, A, El, C, mm.batch, mm.ch,
-                       bb, bk, bn, mm.batch, mm.ch, W2, W2, modes);
+    hipLaunchKernelGGL(mode_row_mix64, dim3((unsigned)((modes + 3) / 4)), dim3(256), 0, s, A, El, C, mm.batch, mm.ch,
+                       (long)W2 * W2, mm.batch, mm.ch, W2, modes);
     return hipGetLastError();
 }
 
@@ -449,7 +470,7 @@ hipError_t forward_impl(bec_fno *f, int B, const float *x, hipStream_t s)
         if ((e = zdft(f, q, hin, f->zf, 0, 0.0f, s)) != hipSuccess) return e;
         if ((e = xydft(f, q, f->zf, f->sp[l], 0, s)) != hipSuccess) return e;
         const unsigned short *El = f->E + (long)l * modes * 4 * W * W;
-        if (B == 1 && 2 * W <= 64) {
+        if (B == 1 && 2 * W == 64) {
             if ((e = mode_mix_row(f, f->sp[l], El, f->tm, 2 * W, mm, modes, s)) != hipSuccess) return e;
         } else if ((e = G(f, f->sp[l], 0, El, 1, f->tm, B, 2 * W, 2 * W, mm.row, mm.ch, m1(2 * W), m1(1), mm.row,
                           mm.ch, (int)modes, mm.batch, m1(4L * W * W), mm.batch, 0.0f, nullptr, 1, nullptr, s)) !=

@@ -209,3 +209,36 @@ class DeviceView:
             raise ValueError("view outside the allocation")
         self.base = base
         self.ptr = C.c_void_p(base.ptr.value + int(offset_bytes))
+
+
+def main(argv=None):
+    """Times the forward pass of the production network (python -m bec_amd.fno):
+    host wall clock over --iters back-to-back passes on one set of device buffers."""
+    import argparse
+    import json
+    import time
+
+    from . import surrogate as S
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--batch", type=int, default=1)
+    a = ap.parse_args(argv)
+    net = FNO(seed=7, max_batch=max(16, a.batch), **S.NET)
+    th = S.sample_params(a.batch, 3)
+    x = np.stack([S.features(S.calc_for(t), t) for t in th])
+    dx = DeviceArray(x.shape, np.float32, x)
+    dy = DeviceArray((a.batch, net.cells, net.cfg.c_out), np.float32)
+    for _ in range(5):
+        net.forward(dx, dy, a.batch)
+    dy.download()
+    t0 = time.perf_counter()
+    for _ in range(a.iters):
+        net.forward(dx, dy, a.batch)
+    dy.download()
+    ms = (time.perf_counter() - t0) / a.iters * 1e3
+    print(json.dumps({"batch": a.batch, "iters": a.iters, "ms_per_forward": ms}), flush=True)
+    net.close()
+
+
+if __name__ == "__main__":
+    main()
