@@ -153,6 +153,32 @@ __global__ __launch_bounds__(256) void mode_row_mix64(const float *A, const unsi
     }
 }
 
+struct Gs2Lattice {
+    int nx, ny, nz;
+    double xmax, ymax, zmax, dx, dy, dz, lambda, mu;
+    float scaled[4];
+};
+
+// One cell of the learned start's input features per thread, in FP64 with the
+// operations in the host construction's order (no contraction), then FP32.
+__global__ void gs2_features(Gs2Lattice g, float *out, long cells)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const int xi = (int)(i % g.nx);
+    const long r = i / g.nx;
+    const int yi = (int)(r % g.ny), zi = (int)(r / g.ny);
+    const double X = -g.xmax + g.dx * xi, Y = -g.ymax + g.dy * yi, Z = -g.zmax + g.dz * zi;
+    const double V = 0.5 * (X * X + Y * Y + Z * Z / (g.lambda * g.lambda));
+    float4 *o = (float4 *)(out + i * 16);
+    o[0] = make_float4((float)sqrt(fmax(g.mu - V, 0.0) / g.mu), (float)(V / g.mu), (float)(X / g.xmax),
+                       (float)(Y / g.ymax));
+    o[1] = make_float4((float)(Z / g.zmax), g.scaled[0], g.scaled[1], g.scaled[2]);
+    o[2] = make_float4(g.scaled[3], 0.f, 0.f, 0.f);
+    o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // E[t] (2W x 2W, BF16) from R[t][i][o][re, im]
 __global__ void expand_modes(const float *R, unsigned short *E, int W, long modes)
 {
@@ -752,6 +778,21 @@ extern "C" int bec_fno_forward(bec_fno *f, int batch, const float *d_x, float *d
     bec::CallerBuffers reg(f, {{d_x, sizeof(float) * batch * f->S * f->c.c_in}});
     BEC_HIP(bec::forward_impl(f, batch, d_x, s));
     BEC_HIP(hipMemcpyAsync(d_y, f->y, sizeof(float) * batch * f->S * f->c.c_out, hipMemcpyDeviceToDevice, s));
+    return BEC_OK;
+}
+
+extern "C" int bec_fno_gs2_features(const bec_calc_params *p, const float scaled[4], float *d_x, void *stream)
+{
+    if (!p || !scaled || !d_x || ((uintptr_t)d_x & 15) != 0) return BEC_ERR_INVALID;
+    if (p->nvx < 1 || p->nvy < 1 || p->nvz < 1 || (long)p->cells != (long)p->nvx * p->nvy * p->nvz) return BEC_ERR_SHAPE;
+    if (!(p->mu > 0.0) || !(p->xmax > 0.0) || !(p->ymax > 0.0) || !(p->zmax > 0.0) || !(p->lambda > 0.0))
+        return BEC_ERR_INVALID;
+    bec::Gs2Lattice g{p->nvx, p->nvy, p->nvz, p->xmax, p->ymax, p->zmax, p->dx, p->dy, p->dz, p->lambda, p->mu,
+                      {scaled[0], scaled[1], scaled[2], scaled[3]}};
+    const long cells = p->cells;
+    hipLaunchKernelGGL(bec::gs2_features, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g,
+                       d_x, cells);
+    BEC_HIP(hipGetLastError());
     return BEC_OK;
 }
 

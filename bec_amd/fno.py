@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .api import BecError, OK, check, lib
+from .api import OK, BecError, CalcParams, check, lib
 
 _P = C.c_void_p
 
@@ -51,6 +51,7 @@ EXPORTS = {
     "bec_fno_get_params": (C.c_int, [_P, _P]),
     "bec_fno_set_params": (C.c_int, [_P, _P]),
     "bec_fno_forward": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "bec_fno_gs2_features": (C.c_int, [C.POINTER(CalcParams), C.POINTER(C.c_float), _P, _P]),
     "bec_fno_loss_backward": (C.c_int, [_P, C.c_int, _P, _P, C.POINTER(C.c_double), _P]),
     "bec_fno_get_grads": (C.c_int, [_P, _P]),
     "bec_fno_adam_step": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _P]),
@@ -199,6 +200,13 @@ class FNO:
         dy = DeviceArray((b, self.cells, self.cfg.c_out), np.float32)
         self.forward(dx, dy, b)
         return dy.download()
+
+
+def gs2_features(calc, scaled, d_x, stream=None):
+    """The learned start's input features of one parameter set, written into the
+    device buffer d_x ([cells][16] FP32) by bec_fno_gs2_features."""
+    s = (C.c_float * 4)(*[float(v) for v in scaled])
+    check(flib().bec_fno_gs2_features(C.byref(calc), s, d_x.ptr, stream), "bec_fno_gs2_features")
 
 
 class DeviceView:

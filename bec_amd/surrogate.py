@@ -218,8 +218,9 @@ def flow(spec):
 
 def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None):
     """Per held-out set: solve from TF and from the network guess (one set per solve).
-    inference_ms is the host wall clock of the whole guess: input features,
-    upload, forward pass, download and conversion to the solver's layout.
+    inference_ms is the host wall clock of the whole guess: input features
+    (built on the device), forward pass, download and conversion to the
+    solver's layout.
     tf_opts / nn_opts: the flow's step and momentum for each start."""
     rows = []
     cells = X.shape[1]
@@ -229,7 +230,7 @@ def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None)
         c = calc_for(t)
         _, r_tf, st_tf = solve([c], tol=tol, **(tf_opts or {}))
         t0 = time.perf_counter()
-        dx.upload(features(c, t)[None])
+        F.gs2_features(c, scaled(t), dx)
         net.forward(dx, dy, 1)
         pred = dy.download()[0]
         g = guesses(c, pred)[None]

@@ -22,6 +22,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "bec_api.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -76,6 +78,15 @@ int bec_fno_get_params(const bec_fno *f, float *host_out);
 int bec_fno_set_params(bec_fno *f, const float *host_in);
 /* Device buffers in the layout [batch][cells][channels] (FP32). */
 int bec_fno_forward(bec_fno *f, int batch, const float *d_x, float *d_y, void *hip_stream);
+/* Input features of the learned two-component ground-state start for one
+ * parameter set, written on the device: d_x[cells][16] (FP32, 16-byte aligned,
+ * p->cells * 16 values; cell = (z nvy + y) nvx + x).  Channels: the
+ * Thomas-Fermi amplitude sqrt(max(mu - V, 0) / mu), V / mu, the coordinates
+ * over their half-extents (x, y, z), the four parameters scaled into [-1, 1]
+ * (`scaled`, given by the caller), and seven zeros.  The values are those of
+ * the host construction in bec_amd/surrogate.py (FP64 arithmetic in the same
+ * order, rounded to FP32).  No counterpart in the reference. */
+int bec_fno_gs2_features(const bec_calc_params *p, const float scaled[4], float *d_x, void *hip_stream);
 /* Loss = mean over batch, cells and c_out of (y - target)^2.  Fills the
  * parameter gradients (bec_fno_get_grads) and returns the loss (FP64 sum). */
 int bec_fno_loss_backward(bec_fno *f, int batch, const float *d_x, const float *d_target, double *loss,

@@ -156,3 +156,4 @@ def test_surrogate_entry_points_validate_on_cpu():
     cfg = fno.FnoConfig(**dict(good, width=24))                      # channel maps need powers of two
     assert L.bec_fno_create(C.byref(cfg), 0, C.byref(h)) == api.ERR_SHAPE
     assert L.bec_fno_forward(None, 1, None, None, None) == api.ERR_INVALID
+    assert L.bec_fno_gs2_features(None, None, None, None) == api.ERR_INVALID

@@ -50,3 +50,21 @@ def test_features_and_targets_layout(bec):
     # the Thomas-Fermi channel peaks at the trap centre
     tf = f[:, 0].reshape(128, 16, 16)
     assert np.unravel_index(np.argmax(tf), tf.shape)[0] in (63, 64)
+
+
+def test_device_features_equal_host_features(bec):
+    """bec_fno_gs2_features writes the same inputs as the host construction."""
+    from bec_amd import fno as F
+    from bec_amd import surrogate as S
+    for t in S.sample_params(4, seed=17):
+        c = S.calc_for(t)
+        host = S.features(c, t)
+        d = F.DeviceArray((1,) + host.shape, np.float32)
+        F.gs2_features(c, S.scaled(t), d)
+        dev = d.download()[0]
+        d.free()
+        exact = float(np.mean(dev == host))
+        print(f"device features equal to the host's: {exact:.6f} of the values")
+        # FP64 arithmetic in the same order; at most one FP32 rounding step apart
+        assert np.all(np.abs(dev - host) <= np.spacing(np.abs(host).astype(np.float32)))
+        assert exact > 0.999
