@@ -246,6 +246,45 @@ def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None)
     return rows
 
 
+def evaluate_batched(net, thetas, tol=1e-6, tf_opts=None, nn_opts=None):
+    """All sets in one batched solve from each start (host wall clock).  The
+    network-started time includes the guesses, made in batches of the
+    network's max_batch (features on the device, forward pass, download,
+    conversion)."""
+    calcs = [calc_for(t) for t in thetas]
+    n, cells = len(calcs), calcs[0].cells
+    solve(calcs[:2], tol=tol, **(tf_opts or {}))     # first-call set-up outside the timing
+    t0 = time.perf_counter()
+    _, r_tf, st_tf = solve(calcs, tol=tol, **(tf_opts or {}))
+    t_tf = (time.perf_counter() - t0) * 1e3
+    b = net.cfg.max_batch
+    dx = F.DeviceArray((b, cells, C_IN), np.float32)
+    dy = F.DeviceArray((b, cells, C_OUT), np.float32)
+    g = np.empty((n, 2, cells), dtype=np.complex128)
+    t0 = time.perf_counter()
+    for s0 in range(0, n, b):
+        m = min(b, n - s0)
+        for j in range(m):
+            F.gs2_features(calcs[s0 + j], scaled(thetas[s0 + j]), F.DeviceView(dx, j * cells * C_IN * 4))
+        net.forward(dx, dy, m)
+        pred = dy.download()
+        for j in range(m):
+            g[s0 + j] = guesses(calcs[s0 + j], pred[j])
+    t_inf = (time.perf_counter() - t0) * 1e3
+    _, r_nn, st_nn = solve(calcs, guess=g, tol=tol, **(nn_opts or {}))
+    t_nn = (time.perf_counter() - t0) * 1e3
+    dx.free()
+    dy.free()
+    it_tf = [r["iterations"] for r in r_tf]
+    it_nn = [r["iterations"] for r in r_nn]
+    return {"sets": n, "converged_all": bool(st_tf == 0 and st_nn == 0),
+            "tf_ms": t_tf, "nn_ms_incl_inference": t_nn, "inference_ms": t_inf, "speedup_time": t_tf / t_nn,
+            "tf_iterations_mean": float(np.mean(it_tf)), "tf_iterations_max": int(max(it_tf)),
+            "nn_iterations_mean": float(np.mean(it_nn)), "nn_iterations_max": int(max(it_nn)),
+            "final_residual_max": {"tf": float(max(r["residual"] for r in r_tf)),
+                                   "nn": float(max(r["residual"] for r in r_nn))}}
+
+
 def summary(rows):
     tf_ms = np.array([r["tf_ms"] for r in rows])
     nn_ms = np.array([r["nn_ms"] + r["inference_ms"] for r in rows])
@@ -280,6 +319,7 @@ def main(argv=None):
     ap.add_argument("--nn", default=NN_FLOW, help="tau:beta of the flow from the network guess")
     ap.add_argument("--nn-sweep", default="", help="extra tau:beta settings of the network-started flow to report")
     ap.add_argument("--legacy", default="", help="tau:beta of both starts for a second comparison on the same network")
+    ap.add_argument("--batched", action="store_true", help="also solve all held-out sets in one batch per start")
     ap.add_argument("--save-net", default=None, help="write the trained parameters (float32 .npy) here")
     ap.add_argument("--load-net", default=None, help="skip training and load the parameters from this .npy")
     a = ap.parse_args(argv)
@@ -319,6 +359,9 @@ def main(argv=None):
         r3 = evaluate(net, thetas[a.train:], X[a.train:], Y[a.train:], log=log, tf_opts=lo, nn_opts=lo)
         res["both_starts_" + a.legacy] = summary(r3)
         log(f"both starts with {a.legacy}: {json.dumps(res['both_starts_' + a.legacy])}")
+    if a.batched:
+        res["batched"] = evaluate_batched(net, thetas[a.train:], tf_opts=tf_opts, nn_opts=nn_opts)
+        log(f"batched: {json.dumps(res['batched'])}")
     if a.out:
         with open(a.out, "w") as f:
             json.dump(dict(res, rows=rows), f)

@@ -229,6 +229,8 @@ def surrogate_summary():
         for k, s in d.items():
             if k.startswith("both_starts_"):
                 out.setdefault(k, cut(s))
+        if "batched" in d:
+            out.setdefault("batched_all_sets", d["batched"])
     return out if out["sources"] else None
 
 
