@@ -478,6 +478,9 @@ class Solver:
         return a, b
 
     def set_state(self, a, b, space=SPACE_X, group=0):
+        a, b = _c128(a), _c128(b)
+        if a.shape != self.shape(group) or b.shape != self.shape(group):
+            raise ValueError(f"state arrays must have shape {self.shape(group)}, got {a.shape} and {b.shape}")
         check(lib().bec_set_state(self._ctx, group, space, _ptr(_c128(a)), _ptr(_c128(b))), "bec_set_state")
 
     @property

@@ -44,11 +44,26 @@ def _compile(src):
     return obj
 
 
-def build(jobs=4, verbose=True):
+def _stale(src, headers_mtime):
+    """An object is stale when it is older than its source or than any header."""
+    obj = os.path.join(OBJ, src + ".o")
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return t < os.path.getmtime(os.path.join(CSRC, src)) or t < headers_mtime
+
+
+def build(jobs=4, verbose=True, incremental=False):
+    """Compile every source (incremental=True: only those whose object is stale) and link."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in DEVICE_SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    headers += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    hm = max(os.path.getmtime(h) for h in headers)
+    todo = [s for s in srcs if not incremental or _stale(s, hm)]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ, s + ".o") for s in srcs]
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs)
     drv = os.path.join(CSRC, "driver.cpp")
     if os.path.exists(drv):
@@ -64,6 +79,7 @@ def build(jobs=4, verbose=True):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--incremental", action="store_true", help="recompile only sources with stale objects")
     a = ap.parse_args()
-    build(a.jobs)
+    build(a.jobs, incremental=a.incremental)
     sys.exit(0)

@@ -154,6 +154,18 @@ int main(int argc, char **argv)
     m.ne = (int)num(kv, "ne", m.ne);
     bec_calc_params c;
     CHECK(bec_fill_derived_params(&m, &c));
+    // the run's length and sampling period are checked before any device work:
+    // a period of 0 would never advance the loop below
+    const double steps_in = num(kv, "steps", 15000), every_in = num(kv, "sample_every", 250);
+    if (!(steps_in >= 0) || steps_in > 2e9 || steps_in != floor(steps_in)) {
+        fprintf(stderr, "steps must be a non-negative integer (got %s)\n", str(kv, "steps", "").c_str());
+        return 2;
+    }
+    if (!(every_in >= 1) || every_in > 2e9 || every_in != floor(every_in)) {
+        fprintf(stderr, "sample_every must be a positive integer (got %s)\n", str(kv, "sample_every", "").c_str());
+        return 2;
+    }
+    const int steps = (int)steps_in, every = (int)every_in;
     printf("mu(TF) = %f\n", c.mu);
 
     const int prec = str(kv, "precision", "fp32") == "fp64" ? BEC_FP64 : BEC_FP32;
@@ -187,8 +199,6 @@ int main(int argc, char **argv)
         CHECK(bec_set_loss_noise(ctx, 1, (uint64_t)num(kv, "loss_noise_seed", (double)(seed + 1)), 0));
     CHECK(bec_init_evolution(ctx, 0, gs.data(), seed, flag(kv, "equilibrate", true) ? 1 : 0));
 
-    const int steps = (int)num(kv, "steps", 15000);
-    const int every = (int)num(kv, "sample_every", 250);
     std::vector<double> t, vis, visg, ntot, na, nb, phase, na_err, nb_err;
     bec_obs ob;
     CHECK(bec_observables(ctx, 0, &ob));

@@ -205,6 +205,17 @@ class FNO:
 def gs2_features(calc, scaled, d_x, stream=None):
     """The learned start's input features of one parameter set, written into the
     device buffer d_x ([cells][16] FP32) by bec_fno_gs2_features."""
+    need = int(calc.cells) * 16 * 4
+    if isinstance(d_x, DeviceView):
+        nbytes = d_x.base.nbytes - (d_x.ptr.value - d_x.base.ptr.value)
+    elif isinstance(d_x, DeviceArray):
+        nbytes = d_x.nbytes
+    else:
+        raise ValueError("d_x must be a DeviceArray or DeviceView")
+    if nbytes < need:
+        raise ValueError(f"d_x holds {nbytes} bytes; the features of {calc.cells} cells need {need}")
+    if len(scaled) != 4:
+        raise ValueError("scaled must hold the four scaled parameters")
     s = (C.c_float * 4)(*[float(v) for v in scaled])
     check(flib().bec_fno_gs2_features(C.byref(calc), s, d_x.ptr, stream), "bec_fno_gs2_features")
 
