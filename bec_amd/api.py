@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libbec_amd.so")
 OK = 0
 ERR_INVALID, ERR_SHAPE, ERR_ALLOC, ERR_HIP, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_STATE, ERR_UNSUPPORTED = range(1, 9)
 FP32, FP64 = 0, 1
-SPACE_X, SPACE_K = 0, 1
+SPACE_X, SPACE_K, SPACE_HYBRID = 0, 1, 2
 PULSE_HALF_PI, PULSE_PI = 0, 1
 FFT_FORWARD, FFT_INVERSE = -1, 1
 GS_SPLIT_STEP, GS_GRADIENT = 0, 1
@@ -455,13 +455,14 @@ class Solver:
         check(lib().bec_trajectory_sums(self._ctx, group, _ptr(out)), "bec_trajectory_sums")
         return out
 
-    def projections(self, alpha=0.0, group=0):
+    def projections(self, alpha=0.0, group=0, static=False):
+        """Density projections (bec_projections)."""
         c = self.groups[group]
         xa = np.empty((c.nvy, c.nvx)); xb = np.empty_like(xa)
         za = np.empty((c.nvy, c.nvz)); zb = np.empty_like(za)
         aa = np.empty(c.nvz); ab = np.empty_like(aa)
-        check(lib().bec_projections(self._ctx, group, alpha, *[_ptr(v) for v in (xa, xb, za, zb, aa, ab)]),
-              "bec_projections")
+        name = "bec_projections"
+        check(getattr(lib(), name)(self._ctx, group, alpha, *[_ptr(v) for v in (xa, xb, za, zb, aa, ab)]), name)
         return dict(xy_a=xa, xy_b=xb, zy_a=za, zy_b=zb, axial_a=aa, axial_b=ab)
 
     def pulse(self, kind=PULSE_HALF_PI):

@@ -79,6 +79,11 @@ hipError_t run_rotation(const DevGroup &g, const double u[8], hipStream_t s);
 // out[cell*2 + c], x-space
 hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, double *out,
                                  hipStream_t s);
+// doubles of the projection workspace of a group: the per-cell sums [cells][2],
+// then xy [2][ny][nx], zy [2][ny][nz], axial [2][nz]
+long projection_doubles(const DevGroup &g);
+// the density projections after halfPiRotate(alpha) into buf (projection_doubles(g) doubles)
+hipError_t run_projections(const DevGroup &g, double alpha, double vac, double *buf, hipStream_t s);
 // plain streaming copy (HBM counter calibration)
 hipError_t run_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t run_convert(int prec_from, const void *in, int prec_to, void *out, long n, hipStream_t s);

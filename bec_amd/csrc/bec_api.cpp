@@ -221,6 +221,10 @@ void group_release(Group &g)
     void *n = d.noise_step;
     dev_free(n);
     d.noise_step = nullptr;
+    void *pj = g.d_proj;
+    dev_free(pj);
+    g.d_proj = nullptr;
+    dev_free(g.d_avg);
 }
 
 int upload_complex(const double *h, void *d, long n, int prec, hipStream_t s)
@@ -764,40 +768,49 @@ extern "C" int bec_calculate_evolution(bec_ctx *ctx, double dt, bec_obs *out)
     return BEC_OK;
 }
 
+namespace bec {
+// The projections of wigner.cu:279-303,388-410 of group `group`, computed on
+// the device into the workspace `buf` of `cap` doubles (BEC_ERR_SHAPE when
+// projection_doubles() exceeds it), then copied into the caller's arrays (each
+// may be NULL): xy [nvy][nvx] (sum over z), zy [nvy][nvz] (sum over x),
+// axial [nvz].  buf NULL selects the group's own workspace, allocated on first use.
+int ctx_projections(bec_ctx *ctx, int group, double alpha, double *buf, size_t cap, double *xy_a, double *xy_b,
+                    double *zy_a, double *zy_b, double *axial_a, double *axial_b)
+{
+    CHECK_GROUP(ctx, group);
+    if (!std::isfinite(alpha)) return BEC_ERR_INVALID;
+    Group &g = ctx->groups[group];
+    if (!g.initialized) return BEC_ERR_STATE;
+    const long need = projection_doubles(g.dev);
+    if (!buf) {
+        if (!g.d_proj) BEC_HIP(dev_alloc((void **)&g.d_proj, sizeof(double) * need));
+        buf = g.d_proj;
+        cap = (size_t)need;
+    }
+    if (need > (long)cap) {
+        set_last_error("projections: the lattice needs " + std::to_string(need) + " doubles of workspace, " +
+                       std::to_string(cap) + " are available");
+        return BEC_ERR_SHAPE;
+    }
+    const long nx = g.calc.nvx, ny = g.calc.nvy, nz = g.calc.nvz;
+    const double vac = g.calc.V / (2.0 * g.calc.dx * g.calc.dy * g.calc.dz);
+    BEC_HIP(run_projections(g.dev, alpha, vac, buf, ctx->stream));
+    const double *xy = buf + 2 * g.cells, *zy = xy + 2 * nx * ny, *ax = zy + 2 * ny * nz;
+    double *dst[6] = {xy_a, xy_b, zy_a, zy_b, axial_a, axial_b};
+    const double *src[6] = {xy, xy + nx * ny, zy, zy + ny * nz, ax, ax + nz};
+    const long len[6] = {nx * ny, nx * ny, ny * nz, ny * nz, nz, nz};
+    for (int i = 0; i < 6; ++i)
+        if (dst[i])
+            BEC_HIP(hipMemcpyAsync(dst[i], src[i], sizeof(double) * len[i], hipMemcpyDeviceToHost, ctx->stream));
+    BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+} // namespace bec
+
 extern "C" int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy_a, double *xy_b, double *zy_a,
                                double *zy_b, double *axial_a, double *axial_b)
 {
-    CHECK_GROUP(ctx, group);
-    Group &g = ctx->groups[group];
-    if (!g.initialized) return BEC_ERR_STATE;
-    const int nx = g.calc.nvx, ny = g.calc.nvy, nz = g.calc.nvz;
-    double *d_out = nullptr;
-    BEC_HIP(dev_alloc((void **)&d_out, sizeof(double) * 2 * g.cells));
-    const double vac = g.calc.V / (2.0 * g.calc.dx * g.calc.dy * g.calc.dz);
-    hipError_t e = run_cell_density_sums(g.dev, alpha, vac, d_out, ctx->stream);
-    std::vector<double> cs(2 * g.cells);
-    if (e == hipSuccess) e = hipMemcpyAsync(cs.data(), d_out, sizeof(double) * 2 * g.cells, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    void *p = d_out;
-    dev_free(p);
-    BEC_HIP(e);
-    // projections of wigner.cu:388-410 (xy: sum over z; zy stored [y][z]: sum
-    // over x) and the axial profile of wigner.cu:279-303 (sum over x, y)
-    for (int c = 0; c < 2; ++c) {
-        double *xy = c ? xy_b : xy_a, *zy = c ? zy_b : zy_a, *ax = c ? axial_b : axial_a;
-        if (xy) std::fill(xy, xy + nx * ny, 0.0);
-        if (zy) std::fill(zy, zy + ny * nz, 0.0);
-        if (ax) std::fill(ax, ax + nz, 0.0);
-        for (int z = 0; z < nz; ++z)
-            for (int y = 0; y < ny; ++y)
-                for (int x = 0; x < nx; ++x) {
-                    double v = cs[2 * (((long)z * ny + y) * nx + x) + c];
-                    if (xy) xy[y * nx + x] += v;
-                    if (zy) zy[y * nz + z] += v;
-                    if (ax) ax[z] += v;
-                }
-    }
-    return BEC_OK;
+    return ctx_projections(ctx, group, alpha, nullptr, 0, xy_a, xy_b, zy_a, zy_b, axial_a, axial_b);
 }
 
 // ---------------------------------------------------------- checkpoint
@@ -805,10 +818,15 @@ extern "C" int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy
 extern "C" int bec_get_state(bec_ctx *ctx, int group, int space, double *a, double *b)
 {
     CHECK_GROUP(ctx, group);
-    if (space != BEC_SPACE_X && space != BEC_SPACE_K) return BEC_ERR_INVALID;
+    if (space != BEC_SPACE_X && space != BEC_SPACE_K && space != BEC_SPACE_HYBRID) return BEC_ERR_INVALID;
     Group &g = ctx->groups[group];
     if (!g.initialized) return BEC_ERR_STATE;
     const long n = g.cells * g.dev.ntraj;
+    if (space == BEC_SPACE_HYBRID) {   // the stored representation, exact (FP32 values widen exactly)
+        int st = a ? download_complex(g.dev.a, a, n, g.dev.prec, ctx->stream) : BEC_OK;
+        if (st == BEC_OK && b) st = download_complex(g.dev.b, b, n, g.dev.prec, ctx->stream);
+        return st;
+    }
     void *tmp = nullptr;
     BEC_HIP(dev_alloc(&tmp, elem_size(g.dev.prec) * n));
     int st = BEC_OK;
@@ -833,7 +851,7 @@ extern "C" int bec_get_state(bec_ctx *ctx, int group, int space, double *a, doub
 extern "C" int bec_set_state(bec_ctx *ctx, int group, int space, const double *a, const double *b)
 {
     CHECK_GROUP(ctx, group);
-    if (space != BEC_SPACE_X && space != BEC_SPACE_K) return BEC_ERR_INVALID;
+    if (space != BEC_SPACE_X && space != BEC_SPACE_K && space != BEC_SPACE_HYBRID) return BEC_ERR_INVALID;
     if (!a || !b) return BEC_ERR_INVALID;
     Group &g = ctx->groups[group];
     const long n = g.cells * g.dev.ntraj;
@@ -841,6 +859,7 @@ extern "C" int bec_set_state(bec_ctx *ctx, int group, int space, const double *a
         void *dst = c ? g.dev.b : g.dev.a;
         int st = upload_complex(c ? b : a, dst, n, g.dev.prec, ctx->stream);
         if (st != BEC_OK) return st;
+        if (space == BEC_SPACE_HYBRID) continue;
         hipError_t e = space == BEC_SPACE_X
                            ? run_xy_fft(g.dev, dst, dst, g.dev.ntraj, +1, 1.0 / (g.calc.nvx * g.calc.nvy), ctx->stream)
                            : run_z_fft(g.dev, dst, dst, g.dev.ntraj, -1, 1.0, ctx->stream);
@@ -861,18 +880,10 @@ extern "C" int bec_calculate_averages(bec_ctx *ctx, int group, bec_avg_report *o
     Group &g = ctx->groups[group];
     if (!g.initialized) return BEC_ERR_STATE;
     const long n = g.cells * g.dev.ntraj;
-    void *ax = nullptr, *bx = nullptr;
-    double *d_out = nullptr;
-    struct Free {
-        void **p[3];
-        ~Free()
-        {
-            for (void **q : p) dev_free(*q);
-        }
-    } fr{{&ax, &bx, (void **)&d_out}};
-    BEC_HIP(dev_alloc(&ax, elem_size(g.dev.prec) * n));
-    BEC_HIP(dev_alloc(&bx, elem_size(g.dev.prec) * n));
-    BEC_HIP(dev_alloc((void **)&d_out, sizeof(double) * 6 * g.cells));
+    const size_t sbytes = elem_size(g.dev.prec) * n;
+    if (!g.d_avg) BEC_HIP(dev_alloc(&g.d_avg, 2 * sbytes + sizeof(double) * 6 * g.cells));
+    void *ax = g.d_avg, *bx = (char *)g.d_avg + sbytes;
+    double *d_out = (double *)((char *)g.d_avg + 2 * sbytes);
     BEC_HIP(run_xy_fft(g.dev, g.dev.a, ax, g.dev.ntraj, -1, 1.0, ctx->stream));
     BEC_HIP(run_xy_fft(g.dev, g.dev.b, bx, g.dev.ntraj, -1, 1.0, ctx->stream));
     BEC_HIP(run_averages(g.dev, ax, bx, d_out, ctx->stream));

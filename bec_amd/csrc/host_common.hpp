@@ -21,6 +21,8 @@ struct Group {
     double *d_partial = nullptr; // per-workgroup partial sums
     long partial_slots = 0;
     double *d_sums = nullptr;    // [ntraj][4]
+    double *d_proj = nullptr;    // projection workspace (projection_doubles), allocated on first use
+    void *d_avg = nullptr;       // calculateAverages workspace: x-space a, b and [cells][6], on first use
     bool initialized = false;
 };
 

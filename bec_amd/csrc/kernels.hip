@@ -474,6 +474,48 @@ hipError_t run_cell_density_sums(const DevGroup &g, double alpha, double vac, do
     });
 }
 
+// Projections of the per-cell sums cs[cell*2 + c] (run_cell_density_sums):
+// out = {xy[2][ny][nx] (sum over z), zy[2][ny][nz] (sum over x),
+// axial[2][nz] (sum over y, then x)}, one thread per output value, each sum in
+// increasing index order (wigner.cu:279-303,388-410).
+__global__ void __launch_bounds__(256) projection_sums_kernel(const double *__restrict__ cs, int nx, int ny, int nz,
+                                                             double *__restrict__ out)
+{
+    const int nxy = nx * ny, nyz = ny * nz;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n_out = 2 * (nxy + nyz + nz);
+    if (i >= n_out) return;
+    double s = 0.0;
+    if (i < 2 * nxy) {
+        const int c = i / nxy, q = i - c * nxy;   // q = y nx + x
+        for (int z = 0; z < nz; ++z) s += cs[((long)z * nxy + q) * 2 + c];
+    } else if (i < 2 * (nxy + nyz)) {
+        const int r = i - 2 * nxy, c = r / nyz, q = r - c * nyz, y = q / nz, z = q - y * nz;
+        for (int x = 0; x < nx; ++x) s += cs[(((long)z * ny + y) * nx + x) * 2 + c];
+    } else {
+        const int r = i - 2 * (nxy + nyz), c = r / nz, z = r - c * nz;
+        for (int q = 0; q < nxy; ++q) s += cs[((long)z * nxy + q) * 2 + c];
+    }
+    out[i] = s;
+}
+
+long projection_doubles(const DevGroup &g)
+{
+    const long cells = (long)g.nx * g.ny * g.nz;
+    return 2 * cells + 2 * ((long)g.nx * g.ny + (long)g.ny * g.nz + g.nz);
+}
+
+hipError_t run_projections(const DevGroup &g, double alpha, double vac, double *buf, hipStream_t s)
+{
+    hipError_t e = run_cell_density_sums(g, alpha, vac, buf, s);
+    if (e != hipSuccess) return e;
+    const long cells = (long)g.nx * g.ny * g.nz;
+    const int n_out = (int)(projection_doubles(g) - 2 * cells);
+    hipLaunchKernelGGL(projection_sums_kernel, dim3((n_out + 255) / 256), dim3(256), 0, s, buf, g.nx, g.ny, g.nz,
+                       buf + 2 * cells);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------- copy
 __global__ void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, long n)
 {

@@ -38,7 +38,10 @@ enum {
 };
 
 enum { BEC_FP32 = 0, BEC_FP64 = 1 };          /* state / FFT / nonlinear precision */
-enum { BEC_SPACE_X = 0, BEC_SPACE_K = 1 };    /* representation for get/set state */
+/* Representation for get/set state: x-space, k-space, or the stored hybrid
+ * H(z, ky, kx) -- the state transformed along x and y only, psi_x = F_xy(-) H,
+ * psi_k = F_z(+) H / nz -- whose round trip is exact (a bit-exact checkpoint). */
+enum { BEC_SPACE_X = 0, BEC_SPACE_K = 1, BEC_SPACE_HYBRID = 2 };
 enum { BEC_PULSE_HALF_PI = 0, BEC_PULSE_PI = 1 };
 enum { BEC_FFT_FORWARD = -1, BEC_FFT_INVERSE = 1 };  /* = CUFFT_FORWARD / CUFFT_INVERSE */
 
@@ -329,7 +332,12 @@ int bec_pulse(bec_ctx *ctx, int kind);
 int bec_pulse_rotation(bec_ctx *ctx, double theta, double phi);
 
 /* Checkpoint: read / write the state of a group in x- or k-space
- * (k-space: psi_k = (1/cells) sum_x psi_x exp(+i k.x), the reference's). */
+ * (k-space: psi_k = (1/cells) sum_x psi_x exp(+i k.x), the reference's), or in
+ * the stored hybrid representation (BEC_SPACE_HYBRID), which restores the
+ * state bit for bit (the host copies of wigner.cu:166,207).  A resumed run
+ * equals the uninterrupted one bit for bit when it also restores the time
+ * (bec_set_time) and, with loss noise on, the noise step
+ * (bec_get_loss_noise_step, then bec_set_loss_noise with that first step). */
 int bec_get_state(bec_ctx *ctx, int group, int space, double *a, double *b);
 int bec_set_state(bec_ctx *ctx, int group, int space, const double *a, const double *b);
 double bec_get_time(const bec_ctx *ctx);
