@@ -168,6 +168,7 @@ EXPORTS = {
     "bec_calculate_evolution": (C.c_int, [_P, C.c_double, C.POINTER(Obs)]),
     "bec_trajectory_sums": (C.c_int, [_P, C.c_int, _P]),
     "bec_projections": (C.c_int, [_P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P]),
+    "bec_projections_static": (C.c_int, [_P, C.c_int, C.c_double, _P, _P, _P, _P, _P, _P]),
     "bec_pulse": (C.c_int, [_P, C.c_int]),
     "bec_pulse_rotation": (C.c_int, [_P, C.c_double, C.c_double]),
     "bec_get_state": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
@@ -456,12 +457,13 @@ class Solver:
         return out
 
     def projections(self, alpha=0.0, group=0, static=False):
-        """Density projections (bec_projections)."""
+        """Density projections (bec_projections; static=True: bec_projections_static,
+        which allocates no device memory)."""
         c = self.groups[group]
         xa = np.empty((c.nvy, c.nvx)); xb = np.empty_like(xa)
         za = np.empty((c.nvy, c.nvz)); zb = np.empty_like(za)
         aa = np.empty(c.nvz); ab = np.empty_like(aa)
-        name = "bec_projections"
+        name = "bec_projections_static" if static else "bec_projections"
         check(getattr(lib(), name)(self._ctx, group, alpha, *[_ptr(v) for v in (xa, xb, za, zb, aa, ab)]), name)
         return dict(xy_a=xa, xy_b=xb, zy_a=za, zy_b=zb, axial_a=aa, axial_b=ab)
 

@@ -22,7 +22,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result", f"--offload-arch={ARCH}",
           "-I" + os.path.join(ROOT, "include")]
 
-DEVICE_SOURCES = ["kernels.hip", "kernels_gs.hip", "gemm.hip", "fno.hip", "kernels2d.hip", "kernels1d.hip"]
+DEVICE_SOURCES = ["kernels.hip", "kernels_gs.hip", "gemm.hip", "fno.hip", "kernels2d.hip", "kernels1d.hip",
+                  "static_scratch.hip"]
 HOST_SOURCES = ["bec_api.cpp", "params.cpp", "gs.cpp", "w2d.cpp", "w1d.cpp"]
 
 

@@ -319,6 +319,16 @@ int bec_trajectory_sums(bec_ctx *ctx, int group, double *sums);
 int bec_projections(bec_ctx *ctx, int group, double alpha, double *xy_a, double *xy_b,
                     double *zy_a, double *zy_b, double *axial_a, double *axial_b);
 
+/* bec_projections with no device allocation on any call: its workspace is a
+ * fixed device array of the library (4.25 MiB per GPU, held until the process
+ * ends; every supported lattice fits, a larger workspace gives BEC_ERR_SHAPE).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Calls that overlap on one GPU share the array and race; each call writes all
+ * of the workspace it reads, so nothing carries over from a previous call. */
+int bec_projections_static(bec_ctx *ctx, int group, double alpha, double *xy_a, double *xy_b,
+                           double *zy_a, double *zy_b, double *axial_a, double *axial_b);
+
 /* applyHalfPiPulse / applyPiPulse (wigner_kernel.cu:320-341) on every group. */
 int bec_pulse(bec_ctx *ctx, int kind);
 
