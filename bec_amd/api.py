@@ -86,7 +86,8 @@ class Params2D(C.Structure):
                 ("dtG", C.c_double), ("ntG", C.c_int), ("itmax", C.c_int),
                 ("dtW", C.c_double), ("ntW", C.c_int), ("n_meas", C.c_int), ("ne", C.c_int),
                 ("n0", C.c_double), ("seed", C.c_uint64), ("noise_dev", C.c_double),
-                ("gs_rule", C.c_int), ("gs_tol", C.c_double), ("gs_max_iter", C.c_int)]
+                ("gs_rule", C.c_int), ("gs_tol", C.c_double), ("gs_max_iter", C.c_int),
+                ("n_atoms", C.c_double), ("gs_tau", C.c_double), ("gs_beta", C.c_double)]
 
 
 class Report2D(C.Structure):
@@ -159,11 +160,13 @@ EXPORTS = {
                                                   _P, _P, C.POINTER(Gs2Report)]),
     "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
     "bec_run_2d": (C.c_int, [C.POINTER(Params2D), _P, _P, _P, _P, _P, C.POINTER(Report2D)]),
+    "bec_fill_2d_params": (C.c_int, [C.POINTER(ModelParams), C.POINTER(Params2D)]),
     "bec_1d_cells": (C.c_int, [C.POINTER(Params1D)]),
     "bec_run_1d": (C.c_int, [C.POINTER(Params1D), _P, _P, _P, _P, _P, _P, C.POINTER(Report1D)]),
     "bec_init_evolution": (C.c_int, [_P, C.c_int, _P, C.c_uint64, C.c_int]),
     "bec_propagate": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, _P]),
     "bec_propagate_graph": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
+    "bec_propagate_prepare": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "bec_observables": (C.c_int, [_P, C.c_int, C.POINTER(Obs)]),
     "bec_calculate_evolution": (C.c_int, [_P, C.c_double, C.POINTER(Obs)]),
     "bec_trajectory_sums": (C.c_int, [_P, C.c_int, _P]),
@@ -312,11 +315,24 @@ def params2d(**kw):
     (its 128 x 128 does not fit one workgroup); keyword arguments override."""
     d = dict(nvx=64, nvy=64, xmax=20.0, ymax=20.0, g11=6.18, g12=6.03, g22=5.88, E=2.5, mu=26.8, Va=0.2, Vb=0.5,
              dtG=1.0 / 31.0, ntG=32, itmax=3, dtW=0.01, ntW=200, n_meas=1, ne=32, n0=1.0, seed=0x5EED,
-             noise_dev=None, gs_rule=0, gs_tol=1e-8, gs_max_iter=100000)
+             noise_dev=None, gs_rule=0, gs_tol=1e-8, gs_max_iter=100000, n_atoms=0.0, gs_tau=0.5, gs_beta=0.6)
     d.update(kw)
     if d["noise_dev"] is None:
         d["noise_dev"] = 1.0 / np.sqrt((2.0 * d["xmax"] / (d["nvx"] - 1)) * (2.0 * d["ymax"] / (d["nvy"] - 1)))
     return Params2D(**d)
+
+
+def params2d_from_model(model, **kw):
+    """The 2D program's parameters for a two-component cloud given in SI units
+    (bec_fill_2d_params): the radial plane of the trap, the axial direction in its
+    oscillator ground state; keyword arguments override the result's fields."""
+    p = Params2D()
+    check(lib().bec_fill_2d_params(C.byref(model), C.byref(p)), "bec_fill_2d_params")
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
 
 
 def run_2d(p, want_avn=True):
@@ -437,6 +453,10 @@ class Solver:
         check(lib().bec_propagate(self._ctx, n_steps, dt, sample_every, _ptr(buf) if ns else None),
               "bec_propagate")
         return buf[:ns]
+
+    def prepare(self, n_steps, sample_every=0, dt=0.0):
+        """Capture the step graphs that propagate(n_steps, dt, sample_every) replays (bec_propagate_prepare)."""
+        check(lib().bec_propagate_prepare(self._ctx, n_steps, dt, sample_every), "bec_propagate_prepare")
 
     def propagate_graph(self, n_steps, steps_per_graph, dt=0.0):
         check(lib().bec_propagate_graph(self._ctx, n_steps, dt, steps_per_graph), "bec_propagate_graph")

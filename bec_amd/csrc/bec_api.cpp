@@ -29,6 +29,16 @@ struct bec_ctx {
     // arguments are device pointers only, so it stays valid when dt changes
     hipGraph_t body_graph = nullptr;
     hipGraphExec_t body_exec = nullptr;
+    // whole segments K/2 N (K N)^(L-1) K/2 of up to SEGMENT_GRAPH_MAX steps,
+    // one graph per segment length L, captured on first use or by
+    // bec_propagate_prepare (a few lengths at most: the sampling period and
+    // the remainder of a call)
+    struct SegGraph {
+        int steps;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<SegGraph> seg;
     bool use_graphs = true;
     // one stream per group: independent groups run as parallel graph branches
     std::vector<hipStream_t> aux;
@@ -37,6 +47,23 @@ struct bec_ctx {
 };
 
 static const int STEP_BLOCK = 32;
+static const int SEGMENT_GRAPH_MAX = 64;   // longest segment captured whole
+static const size_t SEGMENT_GRAPHS_KEPT = 8;
+
+// Drops every captured graph: they hold the kernels' launch configuration
+// and arguments, which schedules and the loss-noise switch change.
+static void drop_graphs(bec_ctx *ctx)
+{
+    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
+    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
+    ctx->body_exec = nullptr;
+    ctx->body_graph = nullptr;
+    for (auto &s : ctx->seg) {
+        (void)hipGraphExecDestroy(s.exec);
+        (void)hipGraphDestroy(s.graph);
+    }
+    ctx->seg.clear();
+}
 
 namespace bec {
 
@@ -362,8 +389,7 @@ extern "C" int bec_destroy(bec_ctx *ctx)
 {
     CHECK_CTX(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
-    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
+    drop_graphs(ctx);
     for (auto &g : ctx->groups) group_release(g);
     void *p = ctx->d_samples;
     dev_free(p);
@@ -485,6 +511,70 @@ static int run_full_steps(bec_ctx *ctx, int count)
     return launch_full_steps(ctx, count);
 }
 
+// The kernels of one segment of L steps of one group: K/2 N (K N)^(L-1) K/2.
+static int launch_segment(Group &g, int L, hipStream_t s)
+{
+    BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, s));
+    BEC_HIP(run_xy_nonlinear(g.dev, s));
+    for (int k = 1; k < L; ++k) {
+        BEC_HIP(run_z_kinetic(g.dev, KT_RT_FULL, s));
+        BEC_HIP(run_xy_nonlinear(g.dev, s));
+    }
+    BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, s));
+    return BEC_OK;
+}
+
+// The graph of a whole segment of L steps (L <= SEGMENT_GRAPH_MAX), captured
+// on first use: one launch per call of up to that many steps.  Several groups
+// run as parallel branches, as in the step-block graph.
+static int segment_graph(bec_ctx *ctx, int L, hipGraphExec_t *out)
+{
+    for (auto &s : ctx->seg)
+        if (s.steps == L) {
+            *out = s.exec;
+            return BEC_OK;
+        }
+    hipGraph_t graph = nullptr;
+    BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int st = BEC_OK;
+    if (ctx->groups.size() == 1) {
+        st = launch_segment(ctx->groups[0], L, ctx->stream);
+    } else {
+        hipError_t e = hipEventRecord(ctx->fork_ev, ctx->stream);
+        for (size_t i = 0; i < ctx->groups.size() && e == hipSuccess && st == BEC_OK; ++i) {
+            e = hipStreamWaitEvent(ctx->aux[i], ctx->fork_ev, 0);
+            if (e == hipSuccess) st = launch_segment(ctx->groups[i], L, ctx->aux[i]);
+            if (e == hipSuccess && st == BEC_OK) e = hipEventRecord(ctx->aux_ev[i], ctx->aux[i]);
+        }
+        for (size_t i = 0; i < ctx->groups.size() && e == hipSuccess && st == BEC_OK; ++i)
+            e = hipStreamWaitEvent(ctx->stream, ctx->aux_ev[i], 0);
+        if (st == BEC_OK && e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            st = hip_status(e);
+        }
+    }
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != BEC_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    BEC_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        BEC_HIP(e);
+    }
+    if (ctx->seg.size() >= SEGMENT_GRAPHS_KEPT) {   // oldest out
+        (void)hipGraphExecDestroy(ctx->seg.front().exec);
+        (void)hipGraphDestroy(ctx->seg.front().graph);
+        ctx->seg.erase(ctx->seg.begin());
+    }
+    ctx->seg.push_back({L, graph, exec});
+    *out = exec;
+    return BEC_OK;
+}
+
 // One call: segments of sample_every steps, each K/2 N (K N)^(L-1) K/2 and a
 // read-only sample at its end; the trailing half step of one step and the
 // leading half step of the next are merged into one K inside a segment.
@@ -495,11 +585,19 @@ static int propagate_impl(bec_ctx *ctx, int n_steps, double dt, int sample_every
     int pos = 0, sample_idx = 0;
     while (pos < n_steps) {
         const int L = sample_every > 0 ? std::min(sample_every, n_steps - pos) : n_steps - pos;
-        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
-        for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
-        int st = run_full_steps(ctx, L - 1);
-        if (st != BEC_OK) return st;
-        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+        int st = BEC_OK;
+        if (ctx->use_graphs && L <= SEGMENT_GRAPH_MAX) {
+            hipGraphExec_t exec = nullptr;
+            st = segment_graph(ctx, L, &exec);
+            if (st != BEC_OK) return st;
+            BEC_HIP(hipGraphLaunch(exec, ctx->stream));
+        } else {
+            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+            for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+            st = run_full_steps(ctx, L - 1);
+            if (st != BEC_OK) return st;
+            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+        }
         if (sample_every > 0 && L == sample_every && d_samples) {
             long off = (long)sample_idx * ntot * 4;
             for (auto &g : ctx->groups) {
@@ -540,6 +638,31 @@ extern "C" int bec_propagate(bec_ctx *ctx, int n_steps, double dt, int sample_ev
         BEC_HIP(hipMemcpyAsync(sums_out, ctx->d_samples, bytes, hipMemcpyDeviceToHost, ctx->stream));
     }
     BEC_HIP(hipStreamSynchronize(ctx->stream));
+    return BEC_OK;
+}
+
+extern "C" int bec_propagate_prepare(bec_ctx *ctx, int n_steps, double dt, int sample_every)
+{
+    CHECK_CTX(ctx);
+    if (n_steps < 0) return BEC_ERR_INVALID;
+    for (auto &g : ctx->groups)
+        if (!g.initialized) return BEC_ERR_STATE;
+    if (n_steps == 0 || !ctx->use_graphs) return BEC_OK;
+    if (dt <= 0) dt = ctx->groups[0].calc.dt_evo;
+    int st = ensure_tables(ctx, dt);
+    if (st != BEC_OK) return st;
+    // the segment lengths the call runs: the period, and the remainder
+    const int first = sample_every > 0 ? std::min(sample_every, n_steps) : n_steps;
+    const int last = sample_every > 0 && n_steps % sample_every ? n_steps % sample_every : first;
+    for (int L : {first, last}) {
+        if (L <= SEGMENT_GRAPH_MAX) {
+            hipGraphExec_t exec = nullptr;
+            st = segment_graph(ctx, L, &exec);
+        } else if (L - 1 >= STEP_BLOCK) {
+            st = ensure_body_graph(ctx);
+        }
+        if (st != BEC_OK) return st;
+    }
     return BEC_OK;
 }
 
@@ -1051,10 +1174,7 @@ extern "C" int bec_set_loss_noise(bec_ctx *ctx, int enable, uint64_t seed, unsig
         d.group_index = (int)i;
     }
     // a captured step graph holds the launches without (or with) the noise
-    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
-    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
-    ctx->body_exec = nullptr;
-    ctx->body_graph = nullptr;
+    drop_graphs(ctx);
     return BEC_OK;
 }
 
@@ -1109,10 +1229,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     g.dev.z_half = sch->z_half;
     g.dev.xy_split = sch->xy_split;
     // a captured step graph holds the old launch configuration
-    if (ctx->body_exec) (void)hipGraphExecDestroy(ctx->body_exec);
-    if (ctx->body_graph) (void)hipGraphDestroy(ctx->body_graph);
-    ctx->body_exec = nullptr;
-    ctx->body_graph = nullptr;
+    drop_graphs(ctx);
     return BEC_OK;
 }
 

@@ -279,6 +279,116 @@ __global__ __launch_bounds__(THREADS) void w2d_ground_state_kernel(W2dParams p, 
     }
 }
 
+// Ground state of n_atoms atoms (gs_rule 2; the program's own loop fixes mu):
+// the preconditioned imaginary-time gradient flow
+//   psi <- psi - tau P (H - mu) psi + beta (psi - psi_prev),  P = 1/(alpha + k^2),
+// renormalised to dx dy sum |psi|^2 = n_atoms after every update, from the
+// Thomas-Fermi profile sqrt(max(mu - r^2/4, 0) / g11), alpha = mu, until the
+// residual ||H psi - mu psi|| / (|mu| ||psi||) is below gs_tol.  One
+// workgroup, the x-space iterate and H psi in LDS, the previous iterate in
+// global memory (pv, one cell per thread, read and written by that thread).
+// The tail (atoms, x-space state, final residual) is that of the program's
+// loop below.
+__global__ __launch_bounds__(THREADS) void w2d_flow_gs_kernel(W2dParams p, double2 *gs_out, double2 *pv,
+                                                             double *rep_out)
+{
+    __shared__ double2 A[MAXN * LDMAX], S[MAXN * LDMAX];
+    __shared__ double2 twx[MAXN / 2], twy[MAXN / 2];
+    __shared__ double scratch[SCRATCH];
+    const Geo g = make_geo(p);
+    make_twiddles(g, twx, twy);
+    const double alpha = p.mu > 0 ? p.mu : 1.0;
+    double s0[2] = {0.0, 0.0};
+    for (int c = threadIdx.x; c < g.n; c += THREADS) {
+        double v = p.mu - r2_of(g, c) / 4.0;
+        v = v > 0 ? sqrt(v / p.g11) : 0.0;
+        A[lds_of(g, c)] = make_double2(v, 0.0);
+        s0[0] += v * v;
+        s0[1] += v > 0 ? 1.0 : 0.0;
+    }
+    block_sum<2>(s0, scratch);
+    if (s0[1] == 0.0) {   // no Thomas-Fermi cloud on the lattice: uniform start
+        for (int c = threadIdx.x; c < g.n; c += THREADS) A[lds_of(g, c)] = make_double2(1.0, 0.0);
+        s0[0] = g.n;
+    }
+    double sc = sqrt(p.n_atoms / (s0[0] * g.dx * g.dy));
+    for (int c = threadIdx.x; c < g.n; c += THREADS) {
+        const int o = lds_of(g, c);
+        A[o] = cs(A[o], sc);
+        pv[c] = A[o];
+    }
+    __syncthreads();
+    int iters = 0;
+    for (;;) {
+        // S = H A
+        for (int c = threadIdx.x; c < g.n; c += THREADS) S[lds_of(g, c)] = A[lds_of(g, c)];
+        __syncthreads();
+        to_k(g, S, nullptr, twx, twy);
+        for (int c = threadIdx.x; c < g.n; c += THREADS) S[lds_of(g, c)] = cs(S[lds_of(g, c)], k2_of(g, c));
+        __syncthreads();
+        fft2d(g, S, nullptr, -1, twx, twy);
+        double s2[2] = {0.0, 0.0};
+        for (int c = threadIdx.x; c < g.n; c += THREADS) {
+            const int o = lds_of(g, c);
+            const double2 ps = A[o];
+            const double v = r2_of(g, c) / 4.0 + p.g11 * nrm(ps);
+            const double2 h = make_double2(S[o].x + v * ps.x, S[o].y + v * ps.y);
+            S[o] = h;
+            s2[0] += ps.x * h.x + ps.y * h.y;
+            s2[1] += nrm(ps);
+        }
+        block_sum<2>(s2, scratch);
+        const double mu = s2[0] / s2[1];
+        double s3[1] = {0.0};
+        for (int c = threadIdx.x; c < g.n; c += THREADS) {
+            const int o = lds_of(g, c);
+            const double2 r = make_double2(S[o].x - mu * A[o].x, S[o].y - mu * A[o].y);
+            S[o] = r;
+            s3[0] += nrm(r);
+        }
+        block_sum<1>(s3, scratch);
+        const double res = sqrt(s3[0] / s2[1]) / fabs(mu);
+        if (!(res == res) || res < p.gs_tol || iters >= p.gs_max_iter) break;   // uniform across the workgroup
+        // S = P (H - mu) A
+        to_k(g, S, nullptr, twx, twy);
+        for (int c = threadIdx.x; c < g.n; c += THREADS)
+            S[lds_of(g, c)] = cs(S[lds_of(g, c)], 1.0 / (alpha + k2_of(g, c)));
+        __syncthreads();
+        fft2d(g, S, nullptr, -1, twx, twy);
+        double s4[1] = {0.0};
+        for (int c = threadIdx.x; c < g.n; c += THREADS) {
+            const int o = lds_of(g, c);
+            const double2 cur = A[o], prev = pv[c], d = S[o];
+            const double2 nw = make_double2(cur.x - p.gs_tau * d.x + p.gs_beta * (cur.x - prev.x),
+                                            cur.y - p.gs_tau * d.y + p.gs_beta * (cur.y - prev.y));
+            pv[c] = cur;
+            A[o] = nw;
+            s4[0] += nrm(nw);
+        }
+        block_sum<1>(s4, scratch);
+        sc = sqrt(p.n_atoms / (s4[0] * g.dx * g.dy));
+        for (int c = threadIdx.x; c < g.n; c += THREADS) A[lds_of(g, c)] = cs(A[lds_of(g, c)], sc);
+        __syncthreads();
+        ++iters;
+    }
+    // the program loop's tail, from k-space: atoms, x-space state, final residual
+    to_k(g, A, nullptr, twx, twy);
+    double s1[1] = {0.0};
+    for (int c = threadIdx.x; c < g.n; c += THREADS) s1[0] += nrm(A[lds_of(g, c)]);
+    block_sum<1>(s1, scratch);
+    fft2d(g, A, nullptr, -1, twx, twy);
+    for (int c = threadIdx.x; c < g.n; c += THREADS) gs_out[c] = A[lds_of(g, c)];
+    __syncthreads();
+    double mu = 0.0;
+    const double r = residual(g, p, A, S, twx, twy, scratch, &mu);
+    if (threadIdx.x == 0) {
+        rep_out[0] = s1[0] * g.dx * g.dy * g.n;
+        rep_out[1] = iters;
+        rep_out[2] = r;
+        rep_out[3] = mu;
+    }
+}
+
 // main_loop (wigner.cpp:278-477) for one trajectory per workgroup.  meas:
 // [traj][n_meas][3] = {N1, N2, visibility}; avn: [traj][cells] x-space
 // density minus the vacuum term after the last step (may be null).
@@ -386,9 +496,14 @@ __global__ __launch_bounds__(THREADS) void w2d_ensemble_kernel(W2dParams p, cons
 
 } // namespace
 
-hipError_t run_w2d_ground_state(const W2dParams &p, double2 *gs_out, double *rep_out, hipStream_t s)
+hipError_t run_w2d_ground_state(const W2dParams &p, double2 *gs_out, double2 *pv, double *rep_out, hipStream_t s)
 {
-    hipLaunchKernelGGL(w2d_ground_state_kernel, dim3(1), dim3(THREADS), 0, s, p, gs_out, rep_out);
+    if (p.gs_rule == 2) {
+        if (!pv) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(w2d_flow_gs_kernel, dim3(1), dim3(THREADS), 0, s, p, gs_out, pv, rep_out);
+    } else {
+        hipLaunchKernelGGL(w2d_ground_state_kernel, dim3(1), dim3(THREADS), 0, s, p, gs_out, rep_out);
+    }
     return hipGetLastError();
 }
 

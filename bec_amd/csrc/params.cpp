@@ -81,3 +81,51 @@ extern "C" int bec_fill_derived_params(const bec_model_params *m, bec_calc_param
     c->ne = m->ne;
     return BEC_OK;
 }
+
+// The 2D program's parameters (programs/bec_wigner_2d.cpp/wigner.cpp:106-136
+// meanings) for the cloud of a 3D parameter set: the radial plane of the trap
+// with the axial direction in its oscillator ground state.  In the program's
+// units (energy hbar w, time 1/w, length l = sqrt(hbar / (2 m w)), w = 2 pi fx)
+// the plane's Hamiltonian is -lap + r^2/4 + g |psi|^2 with
+// dx dy sum |psi|^2 = N, where g = g2D / (hbar w l^2) and
+// g2D = (4 pi hbar^2 a / m) / (sqrt(2 pi) a_z): g = 4 sqrt(2 pi) a / a_z.
+extern "C" int bec_fill_2d_params(const bec_model_params *m, bec_2d_params *p)
+{
+    if (!m || !p) return BEC_ERR_INVALID;
+    if (!(m->m > 0) || !(m->fx > 0) || !(m->fz > 0) || m->N <= 0 || m->ne <= 0) return BEC_ERR_INVALID;
+    if (!pow2(m->nvx) || !pow2(m->nvy)) return BEC_ERR_SHAPE;
+    const double h_bar = 1.054571628e-34;  // wigner.cpp:121
+    const double a0 = 5.2917720859e-11;    // wigner.cpp:122
+    memset(p, 0, sizeof(*p));
+    const double w = 2.0 * M_PI * m->fx;
+    const double a_z = sqrt(h_bar / (m->m * 2.0 * M_PI * m->fz));
+    const double k = 4.0 * sqrt(2.0 * M_PI) * a0 / a_z;
+    p->g11 = k * m->a11;
+    p->g12 = k * m->a12;
+    p->g22 = k * m->a22;
+    p->mu = sqrt(p->g11 * m->N / (2.0 * M_PI));
+    p->nvx = m->nvx;
+    p->nvy = m->nvy;
+    p->xmax = p->ymax = 1.2 * 2.0 * sqrt(p->mu);
+    p->E = 2.0 * M_PI * m->detuning / w;
+    p->Va = m->Va;
+    p->Vb = m->Vb;
+    p->dtG = w * m->dt_steady;
+    p->ntG = 1;
+    p->itmax = m->itmax;
+    p->dtW = w * m->dt_evo;
+    p->ntW = 1;
+    p->n_meas = 1;
+    p->ne = m->ne;
+    p->n0 = 1.0;
+    p->seed = 0x5EED;
+    const double dx = 2.0 * p->xmax / (p->nvx - 1), dy = 2.0 * p->ymax / (p->nvy - 1);
+    p->noise_dev = 0.5 / sqrt(dx * dy);
+    p->gs_rule = 2;
+    p->gs_tol = 1e-8;
+    p->gs_max_iter = 100000;
+    p->n_atoms = m->N;
+    p->gs_tau = 0.5;
+    p->gs_beta = 0.6;
+    return BEC_OK;
+}

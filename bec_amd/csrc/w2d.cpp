@@ -18,9 +18,11 @@ int validate_2d(const bec_2d_params &p)
     if (!pow2_in(p.nvx, 4, bec::W2D_MAX_N) || !pow2_in(p.nvy, 4, bec::W2D_MAX_N)) return BEC_ERR_SHAPE;
     if (!(p.xmax > 0) || !(p.ymax > 0) || p.ne < 1 || p.n_meas < 1 || p.ntW < 0 || p.itmax < 0) return BEC_ERR_INVALID;
     if (!(p.dtG > 0) || !(p.dtW > 0)) return BEC_ERR_INVALID;
-    if (p.gs_rule != 0 && p.gs_rule != 1) return BEC_ERR_INVALID;
+    if (p.gs_rule < 0 || p.gs_rule > 2) return BEC_ERR_INVALID;
     if (p.gs_rule == 0 && p.ntG < 1) return BEC_ERR_INVALID;
-    if (p.gs_rule == 1 && (p.gs_max_iter < 1 || !(p.gs_tol > 0))) return BEC_ERR_INVALID;
+    if (p.gs_rule >= 1 && (p.gs_max_iter < 1 || !(p.gs_tol > 0))) return BEC_ERR_INVALID;
+    if (p.gs_rule == 2 && (!(p.n_atoms > 0) || !(p.gs_tau > 0) || !(p.gs_beta >= 0) || !(p.g11 > 0)))
+        return BEC_ERR_INVALID;
     if (p.ne > 65535) return BEC_ERR_SHAPE;
     return BEC_OK;
 }
@@ -56,9 +58,10 @@ extern "C" int bec_run_2d(const bec_2d_params *pp, double *m, double *var, doubl
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{s};
     Buffers buf;
-    double2 *d_gs = nullptr;
+    double2 *d_gs = nullptr, *d_pv = nullptr;
     double *d_rep = nullptr, *d_meas = nullptr, *d_avn = nullptr;
     BEC_HIP(buf.alloc((void **)&d_gs, sizeof(double2) * cells));
+    if (p.gs_rule == 2) BEC_HIP(buf.alloc((void **)&d_pv, sizeof(double2) * cells));
     BEC_HIP(buf.alloc((void **)&d_rep, sizeof(double) * 4));
     BEC_HIP(buf.alloc((void **)&d_meas, sizeof(double) * 3 * p.ne * p.n_meas));
     if (avn) BEC_HIP(buf.alloc((void **)&d_avn, sizeof(double) * cells * p.ne));
@@ -74,7 +77,7 @@ extern "C" int bec_run_2d(const bec_2d_params *pp, double *m, double *var, doubl
         }
     } eg{e0, e1};
     BEC_HIP(hipEventRecord(e0, s));
-    BEC_HIP(bec::run_w2d_ground_state(p, d_gs, d_rep, s));
+    BEC_HIP(bec::run_w2d_ground_state(p, d_gs, d_pv, d_rep, s));
     BEC_HIP(bec::run_w2d_ensemble(p, d_gs, d_meas, d_avn, s));
     BEC_HIP(hipEventRecord(e1, s));
     std::vector<double> h_rep(4), h_meas(3 * (size_t)p.ne * p.n_meas), h_avn(avn ? cells * p.ne : 0);

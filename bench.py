@@ -110,32 +110,45 @@ SURROGATE_SUMMARIES = [os.path.join(ROOT, "profiles", n) for n in
 
 
 def program_2d(B, with_cpu):
-    """The 2D program (config 0: 64 x 64, complex128, the program's constants, its fixed 31-step
-    ground state, pi/2 pulse, 2000 real-time steps, one trajectory without noise), on the GPU
-    (one workgroup, state in LDS) and, for comparison, the FP64 oracle on the host cores; plus
-    the same evolution for 256 noisy trajectories on the GPU (one workgroup each)."""
-    kw = dict(nvx=64, nvy=64, ne=1, Va=0.0, Vb=0.0, ntW=2000, n_meas=1)
-    B.run_2d(B.params2d(**dict(kw, ntW=10)))                 # first-call setup
+    """Config 0: the 2D program (programs/bec_wigner_2d.cpp) for the Rb-87 cloud of 30k atoms
+    (a11 = 100.4, a22 = 95.0, a12 = 97.66 a0) in the trap's radial plane (bec_fill_2d_params),
+    64 x 64, complex128: ground state of |1> to residual 1e-8 (renormalised gradient flow), pi/2
+    pulse, 2000 steps of 10 us, every step sampled, one trajectory without noise -- on the GPU
+    (one workgroup, state in LDS) and the FP64 oracle on the host cores; plus the same evolution
+    for 256 trajectories with vacuum noise on the GPU (one workgroup each)."""
+    model = B.default_model(N=30000, nvx=64, nvy=64, nvz=2, ne=1, Va=0.0, Vb=0.0, detuning=0.0, dt_evo=1e-5,
+                            dt_steady=1e-5)
+    p = B.params2d_from_model(model, n_meas=2000)
+    B.run_2d(B.params2d_from_model(model, n_meas=10))        # first-call setup
     t0 = time.perf_counter()
-    one = B.run_2d(B.params2d(**kw))
+    one = B.run_2d(p)
     t_gpu = time.perf_counter() - t0
+    pe = B.params2d_from_model(model, n_meas=2000, ne=256, Va=1.0, Vb=1.0)
     t0 = time.perf_counter()
-    ens = B.run_2d(B.params2d(**dict(kw, ne=256, Va=0.2, Vb=0.5)), want_avn=False)
+    ens = B.run_2d(pe, want_avn=False)
     t_ens = time.perf_counter() - t0
-    out = {"workload": "64x64 complex128, 31 ground-state steps, pi/2 pulse, 2000 steps; 1 trajectory "
-                       "(and 256 with the program's noise)",
-           "gpu_ms_1traj": t_gpu * 1e3, "gpu_device_ms_1traj": one["report"]["elapsed_ms"],
+    r = one["report"]
+    out = {"workload": "config 0: Rb-87 30k atoms in the 2D program's units, 64x64 complex128, ground state to "
+                       "residual 1e-8, pi/2 pulse, 2000 sampled steps of 10 us; 1 trajectory (and 256 with "
+                       "vacuum noise)",
+           "gpu_ms_1traj": t_gpu * 1e3, "gpu_device_ms_1traj": r["elapsed_ms"],
+           "ground_state": {"iterations": r["gs_iterations"], "residual": r["gs_residual"], "N0": r["N0"],
+                            "mu": r["gs_mu"]},
+           "visibility_min": float(one["vis"].min()), "visibility_end": float(one["vis"][-1]),
            "gpu_ms_256traj": t_ens * 1e3, "traj_steps_per_s_256": 256 * 2000 / t_ens,
-           "N1_N2_1traj": [float(one["m"][0, 0]), float(one["m"][1, 0])]}
+           "N1_N2_end_1traj": [float(one["m"][0, -1]), float(one["m"][1, -1])],
+           "N1_N2_end_256traj": [float(ens["m"][0, -1]), float(ens["m"][1, -1])]}
     if with_cpu:
         from oracle import oracle as O
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         O.set_threads(threads)
+        op = O.Params2D(**{f: getattr(p, f) for f, _ in O.Params2D._fields_})
         t0 = time.perf_counter()
-        ref = O.run2d(O.params2d(**kw))
+        ref = O.run2d(op)
         t_cpu = time.perf_counter() - t0
         out.update(cpu_ms_1traj=t_cpu * 1e3, cpu_threads=threads, cpu_kind="port",
-                   max_rel_diff_N=float(np.max(np.abs(one["m"] - ref["m"]) / np.abs(ref["m"]).max())))
+                   max_rel_diff_N=float(np.max(np.abs(one["m"] - ref["m"]) / np.abs(ref["m"]).max())),
+                   max_abs_diff_visibility=float(np.max(np.abs(one["vis"] - ref["vis"]))))
     return out
 
 
@@ -212,7 +225,10 @@ def surrogate_summary():
     keep = ("held_out", "converged_all", "tf_iterations_mean", "nn_iterations_mean", "tf_ms_mean",
             "nn_ms_mean_incl_inference", "speedup_mean_time", "speedup_iterations")
     cut = lambda s: {k: s[k] for k in keep if k in s}
-    out = {"sources": [], "commands": []}
+    out = {"measured_by_this_run": False,
+           "note": "copied from the committed summaries below; training the network takes minutes, so the "
+                   "benchmark does not re-run it",
+           "sources": [], "commands": []}
     for path in SURROGATE_SUMMARIES:
         if not os.path.exists(path):
             continue
@@ -234,12 +250,26 @@ def surrogate_summary():
     return out if out["sources"] else None
 
 
+def host_threads():
+    """The host cores this process may use: its CPU affinity, capped by OMP_NUM_THREADS
+    where the environment sets the process's share of a shared host."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
 def cpu_baseline(psi, seconds):
-    """FP64 oracle split-step (same algorithm and seeds) timed on this host's cores."""
+    """FP64 oracle split-step (same algorithm and seeds) timed on this host's cores: OpenMP over
+    trajectories, at least one trajectory per thread."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     O.set_threads(threads)
-    ne = 16
+    ne = max(16, threads)
     c = O.derive(O.model(**config1(ne=ne)))
     a, b = O.init_evolution(c, psi, SEED, equilibrate=True)
     t0 = time.perf_counter()
@@ -251,7 +281,21 @@ def cpu_baseline(psi, seconds):
     el = time.perf_counter() - t0
     return {"value": ne * steps / el, "unit": "trajectory-steps/s", "cores": threads, "kind": "port",
             "sample": f"{ne} trajectories x {steps} steps at 128x16x16 (FP64 oracle, OpenMP {threads} threads, "
-                      f"{cpu_model()}), {el:.1f} s"}
+                      f"{cpu_model()}), {el:.1f} s",
+            "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": cpu_model(),
+            "precision": "FP64 (the oracle has no FP32 build)"}
+
+
+def graph_use(a):
+    """What the timed call replays: a segment of up to 64 steps between samples is one
+    captured graph, a longer one runs 32-step block graphs plus single launches."""
+    if a.no_graphs:
+        return "none: one launch per pass"
+    seg = min(a.sample_every, a.steps) if a.sample_every > 0 else a.steps
+    if seg <= 64:
+        return f"one graph per {seg}-step segment"
+    return f"32-step block graphs inside {seg}-step segments"
 
 
 def main():
@@ -299,6 +343,7 @@ def main():
     s.init_evolution(psi, SEED, equilibrate=True)
     sched = s.autotune(steps=40)[0] if not a.no_autotune else s.schedule()
     s.propagate(a.warmup, sample_every=a.sample_every)
+    s.prepare(a.steps, sample_every=a.sample_every)   # capture the timed call's step graphs, untimed
     s.synchronize()
     t0 = time.perf_counter()
     samples = s.propagate(a.steps, sample_every=a.sample_every)
@@ -362,10 +407,14 @@ def main():
         "value": value, "unit": "trajectory-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": el / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (Philox vacuum noise on the GPU-computed ground state)",
-        "config": {"workload": "64 trajectories x 128x16x16 two-component Rb-87, complex64, N=30000, losses on, "
-                               "Philox seed 0x5EED, dt=40us, sample every 250 steps",
+        "config": {"workload": f"{a.traj} trajectories x 128x16x16 two-component Rb-87, complex64, N=30000, "
+                               f"losses on, Philox seed 0x5EED, dt=40us, FP64 sums sampled every "
+                               f"{a.sample_every} steps ({int(samples.shape[0])} samples in the timed "
+                               f"{a.steps} steps)",
                    "trajectories": a.traj, "lattice": [128, 16, 16], "sample_every": a.sample_every,
-                   "graphs": not a.no_graphs, "schedule": sched},
+                   "samples_taken": int(samples.shape[0]),
+                   "graphs_used": graph_use(a),
+                   "schedule": sched},
         "roofline": roof,
         "cpu_baseline": cpu,
         "hbm_GBs_config_size": {k: a.traj * ALGO_BYTES_PER_TRAJ_STEP / (v * 1e-3) / 1e9

@@ -188,8 +188,12 @@ int bec_state_residual(bec_ctx *ctx, int group, const double *psi_x, double *res
  * (:196-275) then main_loop (:278-477) for ne trajectories, one workgroup per
  * trajectory with its state resident in LDS.  Field meanings are the program's
  * file-scope constants (:106-136) plus the extensions of the measurement loop:
- * n_meas samples after the pulse (the program takes 1), and gs_rule = 1 to stop
- * the ground state at residual < gs_tol instead of after ntG - 1 steps.  The
+ * n_meas samples after the pulse (the program takes 1), gs_rule = 1 to stop
+ * the ground state at residual < gs_tol instead of after ntG - 1 steps, and
+ * gs_rule = 2 for a ground state of n_atoms atoms: the preconditioned gradient
+ * flow of the 3D solver (step gs_tau, momentum gs_beta, preconditioner
+ * 1/(mu + k^2)) from the Thomas-Fermi profile, renormalised after every update,
+ * to residual < gs_tol (the program's loop fixes mu instead of N).  The
  * noise is Philox (counter {cell, trajectory, 0, 0}, key seed) with deviation
  * noise_dev (the program: 1/sqrt(dx dy)).  Lattices: nvx, nvy powers of two
  * from 4 to 64 (the program's 128 x 128 does not fit a workgroup). */
@@ -210,7 +214,26 @@ typedef struct bec_2d_params {
     int gs_rule;
     double gs_tol;
     int gs_max_iter;
+    double n_atoms;   /* gs_rule 2: atoms of the ground state */
+    double gs_tau;    /* gs_rule 2: step of the flow */
+    double gs_beta;   /* gs_rule 2: momentum of the flow */
 } bec_2d_params;
+
+/* The 2D program's parameters for the two-component cloud of a 3D parameter
+ * set (SI units, bec_model_params): the program's isotropic plane is the
+ * trap's radial plane (frequency fx), with the axial direction (fz) frozen in
+ * its oscillator ground state.  Units are those of the program's Hamiltonian
+ * -lap + r^2/4 (wigner.cpp:154-194): energy hbar w, time 1/w, length
+ * sqrt(hbar / (2 m w)), w = 2 pi fx.  Then
+ *   g_ij = 4 sqrt(2 pi) a_ij / a_z,  a_z = sqrt(hbar / (m 2 pi fz)),
+ *   mu = sqrt(g11 N / (2 pi))  (Thomas-Fermi, radius 2 sqrt(mu)),
+ *   xmax = ymax = 1.2 * 2 sqrt(mu),  lattice nvx x nvy,
+ *   dtG = w dt_steady, dtW = w dt_evo, E = 2 pi detuning / w,
+ *   noise deviation: the 3D one, 1/(2 dx dy) per mode,
+ *   ground state gs_rule 2 with n_atoms = N, gs_tol 1e-8, tau 0.5, beta 0.6;
+ * one pulse step (ntW = 1) and one sample (n_meas = 1); ne, Va, Vb and the
+ * itmax of the model.  The caller sets n_meas and the seed. */
+int bec_fill_2d_params(const bec_model_params *m, bec_2d_params *out);
 
 typedef struct bec_2d_report {
     double N0;            /* atoms of the ground state */
@@ -286,6 +309,12 @@ int bec_init_evolution(bec_ctx *ctx, int group, const double *gs_x, uint64_t see
  * state (FP64) are recorded; sums_out (may be NULL) receives
  * [n_steps/sample_every][total trajectories][4] doubles, groups in order. */
 int bec_propagate(bec_ctx *ctx, int n_steps, double dt, int sample_every, double *sums_out);
+
+/* Captures (without running them) the step graphs that bec_propagate with these
+ * arguments replays: each segment of up to 64 steps between samples is one
+ * graph launch, longer ones replay 32-step blocks.  A later bec_propagate then
+ * spends no time on capture.  No-op when graphs are off (bec_set_graphs). */
+int bec_propagate_prepare(bec_ctx *ctx, int n_steps, double dt, int sample_every);
 
 /* Same, captured once into a hipGraph of `steps_per_graph` steps and
  * replayed (launch-overhead-free inner loop).  n_steps must be a multiple of

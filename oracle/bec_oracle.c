@@ -859,6 +859,76 @@ double orc2d_residual(const orc2d_params *p, const orc_c *psi, double *mu_out)
     return sqrt(r / den) / fabs(mu);
 }
 
+/* Ground state of the 2D program's Hamiltonian H = -lap + r^2/4 + g11 |psi|^2
+ * with dx dy sum |psi|^2 = n_atoms (the extension gs_rule = 2; the program's
+ * own loop fixes mu instead, wigner.cpp:196-275): the preconditioned
+ * imaginary-time gradient flow of the 3D solver,
+ *   psi <- psi - tau P (H - mu) psi + beta (psi - psi_prev),  P = 1 / (alpha + k^2),
+ * renormalised after every update, from the Thomas-Fermi profile
+ * sqrt(max(mu - r^2/4, 0) / g11) (uniform when that is empty), alpha = mu,
+ * until the residual ||H psi - mu psi|| / (|mu| ||psi||) is below gs_tol.
+ * Writes the x-space state into a0; returns the iterations taken. */
+static int gs2d_flow(const orc2d_params *p, const tables2d *t, orc_c *a0, double *res_out)
+{
+    int n = t->n, nx = t->nx, ny = t->ny;
+    orc_c *pv = malloc(sizeof(orc_c) * n), *h = malloc(sizeof(orc_c) * n);
+    const double alpha = p->mu > 0 ? p->mu : 1.0;
+    int any = 0;
+    for (int s = 0; s < n; s++) {
+        double v = t->pG[s];   /* mu - r^2/4 */
+        v = v > 0 ? sqrt(v / p->g11) : 0.0;
+        any |= v > 0;
+        a0[s] = cmk(v, 0.0);
+    }
+    if (!any)
+        for (int s = 0; s < n; s++) a0[s] = cmk(1.0, 0.0);
+    double nn = 0;
+    for (int s = 0; s < n; s++) nn += cnorm(a0[s]);
+    double sc = sqrt(p->n_atoms / (nn * t->dx * t->dy));
+    for (int s = 0; s < n; s++) { a0[s] = cscale(a0[s], sc); pv[s] = a0[s]; }
+    int it = 0;
+    double res = 0;
+    for (;;) {
+        /* h = H a0 */
+        memcpy(h, a0, sizeof(orc_c) * n);
+        to_k2d(h, nx, ny);
+        for (int s = 0; s < n; s++) h[s] = cscale(h[s], -t->prop[s]);
+        fft2d(h, nx, ny, -1);
+        double num = 0, den = 0;
+        for (int s = 0; s < n; s++) {
+            double v = (p->mu - t->pG[s]) + p->g11 * cnorm(a0[s]);
+            h[s] = cadd(h[s], cscale(a0[s], v));
+            num += a0[s].re * h[s].re + a0[s].im * h[s].im;
+            den += cnorm(a0[s]);
+        }
+        double mu = num / den, r = 0;
+        for (int s = 0; s < n; s++) {
+            h[s] = csub(h[s], cscale(a0[s], mu));
+            r += cnorm(h[s]);
+        }
+        res = sqrt(r / den) / fabs(mu);
+        if (!(res == res) || res < p->gs_tol || it >= p->gs_max_iter) break;
+        /* d = P (H - mu) a0 */
+        to_k2d(h, nx, ny);
+        for (int s = 0; s < n; s++) h[s] = cscale(h[s], 1.0 / (alpha - t->prop[s]));
+        fft2d(h, nx, ny, -1);
+        nn = 0;
+        for (int s = 0; s < n; s++) {
+            orc_c cur = a0[s];
+            orc_c nw = cadd(csub(cur, cscale(h[s], p->gs_tau)), cscale(csub(cur, pv[s]), p->gs_beta));
+            pv[s] = cur;
+            a0[s] = nw;
+            nn += cnorm(nw);
+        }
+        sc = sqrt(p->n_atoms / (nn * t->dx * t->dy));
+        for (int s = 0; s < n; s++) a0[s] = cscale(a0[s], sc);
+        it++;
+    }
+    free(pv); free(h);
+    if (res_out) *res_out = res;
+    return it;
+}
+
 /* calculate_steady_state (wigner.cpp:196-275) + main_loop (wigner.cpp:278-477) */
 int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c *gs_out,
               double *avn, orc2d_report *rep)
@@ -876,7 +946,11 @@ int orc2d_run(const orc2d_params *p, double *m, double *var, double *vis, orc_c 
     to_k2d(a, nx, ny);
     int iters = 0;
     double res = -1;
-    int total = (p->gs_rule == 0) ? p->ntG - 1 : p->gs_max_iter;
+    int total = (p->gs_rule == 0) ? p->ntG - 1 : p->gs_rule == 1 ? p->gs_max_iter : 0;
+    if (p->gs_rule == 2) {   /* the flow works in x-space; a is brought back to k-space for the shared tail */
+        iters = gs2d_flow(p, &t, a, &res);
+        to_k2d(a, nx, ny);
+    }
     for (int it = 0; it < total; it++) {
         for (int s = 0; s < n; s++) a[s] = cmul(a[s], t.propG[s]);
         fft2d(a, nx, ny, -1);

@@ -164,8 +164,11 @@ typedef struct {
     double n0;
     uint64_t seed;
     double noise_dev;             /* reference: 1/sqrt(dx*dy) */
-    int gs_rule;                  /* 0: fixed ntG-1 steps (reference); 1: to residual gs_tol */
+    int gs_rule;                  /* 0: fixed ntG-1 steps (reference); 1: to residual gs_tol;
+                                     2: gradient flow renormalised to n_atoms, to residual gs_tol */
     double gs_tol; int gs_max_iter;
+    double n_atoms;               /* gs_rule 2: atoms of the ground state */
+    double gs_tau, gs_beta;       /* gs_rule 2: step and momentum of the flow */
 } orc2d_params;
 
 typedef struct {

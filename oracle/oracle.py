@@ -60,7 +60,8 @@ class Params2D(C.Structure):
                 ("dtG", C.c_double), ("ntG", C.c_int), ("itmax", C.c_int),
                 ("dtW", C.c_double), ("ntW", C.c_int), ("n_meas", C.c_int), ("ne", C.c_int),
                 ("n0", C.c_double), ("seed", C.c_uint64), ("noise_dev", C.c_double),
-                ("gs_rule", C.c_int), ("gs_tol", C.c_double), ("gs_max_iter", C.c_int)]
+                ("gs_rule", C.c_int), ("gs_tol", C.c_double), ("gs_max_iter", C.c_int),
+                ("n_atoms", C.c_double), ("gs_tau", C.c_double), ("gs_beta", C.c_double)]
 
 
 class Report2D(C.Structure):
@@ -460,7 +461,7 @@ def params2d(**kw):
     d = dict(nvx=128, nvy=128, xmax=20.0, ymax=20.0, g11=6.18, g12=6.03, g22=5.88, E=2.5,
              mu=26.8, Va=0.2, Vb=0.5, dtG=1.0 / 31.0, ntG=32, itmax=3, dtW=0.01, ntW=200,
              n_meas=1, ne=32, n0=1.0, seed=0x5EED, noise_dev=None, gs_rule=0, gs_tol=1e-8,
-             gs_max_iter=100000)
+             gs_max_iter=100000, n_atoms=0.0, gs_tau=0.5, gs_beta=0.6)
     d.update(kw)
     if d["noise_dev"] is None:
         dx = 2.0 * d["xmax"] / (d["nvx"] - 1)

@@ -10,27 +10,28 @@ pytestmark = pytest.mark.gpu
 SEED = 0x5EED
 
 
-def _model(ne, nxy=32, nz=128):
+def _model(ne, nxy=32, nz=128, dt=4e-5):
     return dict(N=30000, nvx=nxy, nvy=nxy, nvz=nz, ne=ne, gamma111=5.4e-30, gamma12=0.78e-13, gamma22=1.5e-14,
-                detuning=0.0, dt_evo=4e-5, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
+                detuning=0.0, dt_evo=dt, dt_steady=1e-5, t_equilib=0.0, Va=1.0, Vb=1.0, itmax=1)
 
 
-# (lattice, FP32 state bar, FP32 observable bar) after 200 steps.  At 128x32x32
-# the largest kinetic phase per 40 us step is ~4.6 rad (> pi), where the split
-# step amplifies any perturbation; FP32 rounding is amplified accordingly.
-CASES = [(16, 5e-4, 1e-4), (32, 3e-3, 2e-3)]
+# (lattice, step, FP32 state bar, FP32 observable bar) after 200 steps.  The
+# 128x32x32 lattice runs at 20 us: at 40 us its largest kinetic phase per step
+# (5.2 rad) lies in the split step's unstable band and any perturbation, FP64
+# rounding included, grows (tests/test_oracle_stability.py).
+CASES = [(16, 4e-5, 5e-4, 1e-4), (32, 2e-5, 5e-4, 1e-4)]
 
 
-@pytest.mark.parametrize("nxy,tol_state32,tol_obs32", CASES)
-def test_precision_split_vs_fp64_oracle(bec, oracle, nxy, tol_state32, tol_obs32):
-    c1 = bec.derive(bec.default_model(**_model(1, nxy=nxy)))
+@pytest.mark.parametrize("nxy,dt,tol_state32,tol_obs32", CASES)
+def test_precision_split_vs_fp64_oracle(bec, oracle, nxy, dt, tol_state32, tol_obs32):
+    c1 = bec.derive(bec.default_model(**_model(1, nxy=nxy, dt=dt)))
     with bec.Solver(c1, precision=bec.FP64) as g:
         gs, rep, st = g.calculate_steady_state(method=bec.GS_GRADIENT, rule=bec.GS_RULE_RESIDUAL, tol=1e-8,
                                                precision=bec.FP64, max_iter=20000)
     assert st == 0
     ne, steps = 2, 200
-    oc = oracle.derive(oracle.model(**_model(ne, nxy=nxy)))
-    c = bec.derive(bec.default_model(**_model(ne, nxy=nxy)))
+    oc = oracle.derive(oracle.model(**_model(ne, nxy=nxy, dt=dt)))
+    c = bec.derive(bec.default_model(**_model(ne, nxy=nxy, dt=dt)))
     oa, ob = oracle.init_evolution(oc, gs, SEED, equilibrate=True)
     oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, steps)
     ox, oy = oracle.to_xspace(oc, oa), oracle.to_xspace(oc, ob)
@@ -71,3 +72,39 @@ def test_fp32_visibility_error_over_long_run(bec):
     assert e["visibility_grid"]["max_abs"] <= 1e-4, e
     assert e["visibility"]["max_abs"] <= 1e-4, e
     assert e["N_total"]["max_rel"] <= 5e-4, e
+
+
+def _config1_meanfield_20000(bec):
+    from bec_amd import precision_study as P
+    return P.main(["--traj", "8", "--steps", "20000", "--sample-every", "250", "--nxy", "16", "--va", "0",
+                   "--vb", "0"])
+
+
+@pytest.fixture(scope="module")
+def long_run(bec):
+    return _config1_meanfield_20000(bec)
+
+
+@pytest.mark.xfail(strict=True, reason="FP32 misses the 1e-4 visibility target over 20000 steps: the "
+                   "transforms' rounding drifts the norm by ~3e-8 per step and the visibility follows "
+                   "the atom number (measured 1.9e-4); the all-FP64 context meets it")
+def test_fp32_visibility_within_1e4_over_20000_steps(long_run):
+    """Config-1 physics without noise, 20000 steps (0.8 s): the 1e-4 visibility
+    target against the all-FP64 GPU run (which equals the FP64 oracle to 1e-11
+    step by step)."""
+    e = long_run["fp32_vs_fp64_gpu"]
+    assert e["visibility"]["max_abs"] <= 1e-4, e
+
+
+def test_fp32_error_over_20000_steps_stays_at_its_measured_level(long_run):
+    """The same run held to what FP32 reaches, so that a regression shows: the
+    visibility within 2.5e-4 (measured 1.9e-4; on the angle grid 1e-3, measured
+    5.8e-4: a drifted phase moves the grid's maximum to the next angle), the
+    atom number within 1e-3 of N (measured 3e-4), the relative phase within
+    5e-3 rad (measured 4.4e-3)."""
+    e = long_run["fp32_vs_fp64_gpu"]
+    print("FP32 vs FP64 over 20000 steps:", e, flush=True)
+    assert e["visibility"]["max_abs"] <= 2.5e-4, e
+    assert e["visibility_grid"]["max_abs"] <= 1e-3, e
+    assert e["N_total"]["max_rel"] <= 1e-3, e
+    assert e["phase"]["max_abs"] <= 5e-3, e
