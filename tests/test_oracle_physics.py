@@ -238,3 +238,46 @@ def test_observables_from_sums_match_per_cell(oracle):
         cr = math.sin(alpha) * S[2] + math.cos(alpha) * S[3]
         assert n1 == pytest.approx(0.5 * (S[0] + S[1]) + cr - vac, rel=1e-12)
         assert n2 == pytest.approx(0.5 * (S[0] + S[1]) - cr - vac, rel=1e-12)
+
+
+def _step(oracle, c, a_k, b_k, dt, kin_sign):
+    """One real-time step K/2 N K/2 assembled from the oracle's pieces; kin_sign = -1
+    plants a flipped kinetic phase (the mistake the stationarity test must catch)."""
+    a_k, b_k = oracle.kinetic_rt(c, a_k, b_k, kin_sign * dt)   # the half step exp(+i (k^2/2) dt/2)
+    a_x, b_x = oracle.to_xspace(c, a_k), oracle.to_xspace(c, b_k)
+    a_x, b_x = oracle.nonlinear_rt(c, a_x, b_x, dt)
+    a_k, b_k = oracle.to_kspace(c, a_x), oracle.to_kspace(c, b_x)
+    return oracle.kinetic_rt(c, a_k, b_k, kin_sign * dt)
+
+
+def test_ground_state_is_stationary_in_real_time_3d(oracle):
+    """The interacting 3D ground state (|1> only, g11 at the 30k-atom configuration,
+    no pulse, no losses, no noise) keeps its density under the real-time step:
+    e^{+i (k^2/2) dt/2} and e^{+i (V + g n) dt} (quirk Q1) are the same Hamiltonian
+    the imaginary-time solvers relax with.  The oracle's propagate keeps |a|^2
+    to 2e-4 of its peak over 200 steps (8 ms, most of a radial period); with the
+    kinetic sign flipped the density moves by more than 10% of its peak."""
+    c = oracle.derive(oracle.model(N=30000, nvx=16, nvy=16, nvz=128, ne=1, a12=0.0, Va=0.0, Vb=0.0,
+                                   gamma111=0.0, gamma12=0.0, gamma22=0.0, detuning=0.0))
+    # a12 = 0 decouples the two-component solve: its |1> is the one-component ground state of N/2 atoms
+    a, b, rep = oracle.ground_state2(c, frac_a=0.5, tol=1e-9, max_iter=5000)
+    assert rep.converged
+    assert oracle.residual(c, a)[0] < 1.5e-9
+    d0 = np.abs(a) ** 2
+    zero = np.zeros_like(a)
+    a_k, b_k = oracle.to_kspace(c, a), oracle.to_kspace(c, zero)
+    ok_k = oracle.propagate(c, a_k, b_k, c.dt_evo, 200)[0]
+    moved_ok = np.abs(np.abs(oracle.to_xspace(c, ok_k)) ** 2 - d0).max() / d0.max()
+    fa, fb = a_k, b_k
+    for _ in range(200):
+        fa, fb = _step(oracle, c, fa, fb, c.dt_evo, -1)
+    moved_flip = np.abs(np.abs(oracle.to_xspace(c, fa)) ** 2 - d0).max() / d0.max()
+    # the assembled step with the right sign is the oracle's propagate
+    ra, rb = a_k, b_k
+    for _ in range(3):
+        ra, rb = _step(oracle, c, ra, rb, c.dt_evo, +1)
+    pa = oracle.propagate(c, a_k, b_k, c.dt_evo, 3)[0]
+    print(f"density change over 200 steps: {moved_ok:.2e} (right sign), {moved_flip:.2e} (flipped kinetic sign)")
+    assert np.abs(ra - pa).max() < 1e-12 * np.abs(pa).max()
+    assert moved_ok < 2e-4
+    assert moved_flip > 0.1
