@@ -188,6 +188,7 @@ EXPORTS = {
     "bec_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "bec_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "bec_set_graphs": (C.c_int, [_P, C.c_int]),
+    "bec_set_multi_launch": (C.c_int, [_P, C.c_int]),
     "bec_set_loss_noise": (C.c_int, [_P, C.c_int, C.c_uint64, C.c_uint]),
     "bec_calculate_averages": (C.c_int, [_P, C.c_int, C.POINTER(AvgReport)]),
     "bec_get_loss_noise_step": (C.c_int, [_P, C.c_int, C.POINTER(C.c_uint)]),
@@ -547,6 +548,10 @@ class Solver:
 
     def set_graphs(self, enable):
         check(lib().bec_set_graphs(self._ctx, 1 if enable else 0), "bec_set_graphs")
+
+    def set_multi_launch(self, enable=True):
+        """Step every group in one launch per pass (bec_set_multi_launch)."""
+        check(lib().bec_set_multi_launch(self._ctx, 1 if enable else 0), "bec_set_multi_launch")
 
     def set_loss_noise(self, enable=True, seed=0x1055, first_step=0):
         """Add (or remove) the loss-induced Wigner noise to every nonlinear step (bec_set_loss_noise)."""

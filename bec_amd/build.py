@@ -23,7 +23,7 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result", f"--offlo
           "-I" + os.path.join(ROOT, "include")]
 
 DEVICE_SOURCES = ["kernels.hip", "kernels_gs.hip", "gemm.hip", "fno.hip", "kernels2d.hip", "kernels1d.hip",
-                  "static_scratch.hip"]
+                  "static_scratch.hip", "multi.hip"]
 HOST_SOURCES = ["bec_api.cpp", "params.cpp", "gs.cpp", "w2d.cpp", "w1d.cpp"]
 
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "host_common.hpp"
+#include "multi.hpp"
 
 struct bec_ctx {
     int prec = 0;
@@ -44,6 +45,11 @@ struct bec_ctx {
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> aux_ev;
     hipEvent_t fork_ev = nullptr;
+    // bec_set_multi_launch: every group in one launch per pass (multi.hip);
+    // the plan holds the groups' device pointers and is rebuilt after a change
+    bool multi = false;
+    bool multi_stale = true;
+    bec::MultiPlan multi_plan{};
 };
 
 static const int STEP_BLOCK = 32;
@@ -63,6 +69,7 @@ static void drop_graphs(bec_ctx *ctx)
         (void)hipGraphDestroy(s.graph);
     }
     ctx->seg.clear();
+    ctx->multi_stale = true;
 }
 
 namespace bec {
@@ -447,12 +454,59 @@ static int total_traj(const bec_ctx *ctx)
     return n;
 }
 
+static int ensure_multi_plan(bec_ctx *ctx)
+{
+    if (!ctx->multi_stale) return BEC_OK;
+    std::vector<const DevGroup *> devs;
+    for (auto &g : ctx->groups) devs.push_back(&g.dev);
+    if (!multi_plan(devs.data(), (int)devs.size(), &ctx->multi_plan)) {
+        set_last_error("these groups cannot share one launch (FP32 states, at most 8 groups, the same itmax == 1 "
+                       "and loss-noise setting)");
+        return BEC_ERR_UNSUPPORTED;
+    }
+    ctx->multi_stale = false;
+    return BEC_OK;
+}
+
+// The kinetic z pass (table variant) of every group on stream s: one launch
+// over all of them when the context steps its groups together, else one each.
+static int z_pass_all(bec_ctx *ctx, int variant, hipStream_t s)
+{
+    if (ctx->multi) {
+        int st = ensure_multi_plan(ctx);
+        if (st != BEC_OK) return st;
+        BEC_HIP(run_z_multi(ctx->multi_plan, variant, s));
+        return BEC_OK;
+    }
+    for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, variant, s));
+    return BEC_OK;
+}
+
+// the nonlinear xy pass of every group, as z_pass_all
+static int xy_pass_all(bec_ctx *ctx, hipStream_t s)
+{
+    if (ctx->multi) {
+        int st = ensure_multi_plan(ctx);
+        if (st != BEC_OK) return st;
+        BEC_HIP(run_xy_multi(ctx->multi_plan, s));
+        return BEC_OK;
+    }
+    for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, s));
+    return BEC_OK;
+}
+
+#define BEC_TRY(x)                                                                                                     \
+    do {                                                                                                               \
+        const int st_ = (x);                                                                                           \
+        if (st_ != BEC_OK) return st_;                                                                                 \
+    } while (0)
+
 // count full steps [K N] (merged kinetic step + nonlinear step) on every group
 static int launch_full_steps(bec_ctx *ctx, int count)
 {
     for (int i = 0; i < count; ++i) {
-        for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_FULL, ctx->stream));
-        for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
+        BEC_TRY(z_pass_all(ctx, KT_RT_FULL, ctx->stream));
+        BEC_TRY(xy_pass_all(ctx, ctx->stream));
     }
     return BEC_OK;
 }
@@ -462,7 +516,7 @@ static int launch_full_steps(bec_ctx *ctx, int count)
 // the graph holds one independent branch per group (uneven grids overlap).
 static int capture_branches(bec_ctx *ctx)
 {
-    if (ctx->groups.size() == 1) return launch_full_steps(ctx, STEP_BLOCK);
+    if (ctx->groups.size() == 1 || ctx->multi) return launch_full_steps(ctx, STEP_BLOCK);
     BEC_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
     for (size_t i = 0; i < ctx->groups.size(); ++i) {
         hipStream_t s = ctx->aux[i];
@@ -481,6 +535,7 @@ static int capture_branches(bec_ctx *ctx)
 static int ensure_body_graph(bec_ctx *ctx)
 {
     if (ctx->body_exec) return BEC_OK;
+    if (ctx->multi) BEC_TRY(ensure_multi_plan(ctx));   // outside the capture
     hipGraph_t graph = nullptr;
     BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int st = capture_branches(ctx);
@@ -524,9 +579,22 @@ static int launch_segment(Group &g, int L, hipStream_t s)
     return BEC_OK;
 }
 
+// the same segment on every group, one launch per pass over all of them
+static int launch_segment_multi(bec_ctx *ctx, int L, hipStream_t s)
+{
+    BEC_TRY(z_pass_all(ctx, KT_RT_HALF, s));
+    BEC_TRY(xy_pass_all(ctx, s));
+    for (int k = 1; k < L; ++k) {
+        BEC_TRY(z_pass_all(ctx, KT_RT_FULL, s));
+        BEC_TRY(xy_pass_all(ctx, s));
+    }
+    return z_pass_all(ctx, KT_RT_HALF, s);
+}
+
 // The graph of a whole segment of L steps (L <= SEGMENT_GRAPH_MAX), captured
 // on first use: one launch per call of up to that many steps.  Several groups
-// run as parallel branches, as in the step-block graph.
+// run as parallel branches, as in the step-block graph, or, when the context
+// steps them together, as one launch per pass.
 static int segment_graph(bec_ctx *ctx, int L, hipGraphExec_t *out)
 {
     for (auto &s : ctx->seg)
@@ -534,10 +602,13 @@ static int segment_graph(bec_ctx *ctx, int L, hipGraphExec_t *out)
             *out = s.exec;
             return BEC_OK;
         }
+    if (ctx->multi) BEC_TRY(ensure_multi_plan(ctx));   // outside the capture
     hipGraph_t graph = nullptr;
     BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     int st = BEC_OK;
-    if (ctx->groups.size() == 1) {
+    if (ctx->multi) {
+        st = launch_segment_multi(ctx, L, ctx->stream);
+    } else if (ctx->groups.size() == 1) {
         st = launch_segment(ctx->groups[0], L, ctx->stream);
     } else {
         hipError_t e = hipEventRecord(ctx->fork_ev, ctx->stream);
@@ -592,11 +663,10 @@ static int propagate_impl(bec_ctx *ctx, int n_steps, double dt, int sample_every
             if (st != BEC_OK) return st;
             BEC_HIP(hipGraphLaunch(exec, ctx->stream));
         } else {
-            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
-            for (auto &g : ctx->groups) BEC_HIP(run_xy_nonlinear(g.dev, ctx->stream));
-            st = run_full_steps(ctx, L - 1);
-            if (st != BEC_OK) return st;
-            for (auto &g : ctx->groups) BEC_HIP(run_z_kinetic(g.dev, KT_RT_HALF, ctx->stream));
+            BEC_TRY(z_pass_all(ctx, KT_RT_HALF, ctx->stream));
+            BEC_TRY(xy_pass_all(ctx, ctx->stream));
+            BEC_TRY(run_full_steps(ctx, L - 1));
+            BEC_TRY(z_pass_all(ctx, KT_RT_HALF, ctx->stream));
         }
         if (sample_every > 0 && L == sample_every && d_samples) {
             long off = (long)sample_idx * ntot * 4;
@@ -676,6 +746,7 @@ extern "C" int bec_propagate_graph(bec_ctx *ctx, int n_steps, double dt, int ste
     if (dt <= 0) dt = ctx->groups[0].calc.dt_evo;
     int st = ensure_tables(ctx, dt);
     if (st != BEC_OK) return st;
+    if (ctx->multi) BEC_TRY(ensure_multi_plan(ctx));   // outside the capture
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     BEC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -1193,6 +1264,18 @@ extern "C" int bec_set_graphs(bec_ctx *ctx, int enable)
 {
     CHECK_CTX(ctx);
     ctx->use_graphs = enable != 0;
+    return BEC_OK;
+}
+
+extern "C" int bec_set_multi_launch(bec_ctx *ctx, int enable)
+{
+    CHECK_CTX(ctx);
+    drop_graphs(ctx);
+    ctx->multi = false;
+    if (!enable) return BEC_OK;
+    int st = ensure_multi_plan(ctx);
+    if (st != BEC_OK) return st;
+    ctx->multi = true;
     return BEC_OK;
 }
 

@@ -95,6 +95,14 @@ __device__ __forceinline__ void row2col(cpx<T> *tile, int t, cpx<T> (&v)[N])
     __syncthreads();
 }
 
+// A copy of v that the compiler cannot see through: values derived from it
+// are recomputed where they are used rather than hoisted out of a loop.
+__device__ __forceinline__ int opaque_index(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 struct XyGeom {
     int nz;          // planes per trajectory
     long n_planes;   // planes in the launch
@@ -110,16 +118,33 @@ struct XyGeom {
 //   const cpx<T>* src(int c); cpx<T>* dst(int c);
 //   void apply(cpx<T> (&rows)[NC][N], int y, int z, int traj, long row_index, double* sums)
 // MINW: minimum waves per SIMD the register allocation must allow.
+// One tile (the 64/N planes blk*PPW ..) of the xy pass, by one 64-lane
+// workgroup, exchanging through `lds` (PPW * N * (N + 1) complex values).
+// lane is threadIdx.x; a caller that loops over tiles passes it through an
+// opaque copy (opaque_index), so that the per-lane LDS addresses stay inside
+// its loop instead of being hoisted out and held in registers.
+template <typename T, int N, class Op, bool NT>
+__device__ __forceinline__ void xy_tile(const Op &op, const XyGeom &g, double *partial, long blk, cpx<T> *lds,
+                                        int lane);
+
 template <typename T, int N, class Op, bool NT = false, int MINW = 1>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 xy_kernel(Op op, XyGeom g, double *partial)
 {
     constexpr int PPW = 64 / N;
-    constexpr int NC = Op::NC;
     __shared__ cpx<T> lds[PPW * N * (N + 1)];
-    const int lane = threadIdx.x;
-    const int pl = lane / N, t = lane % N;
     const long blk = g.xcd ? xcd_remap(blockIdx.x, g.nz / PPW) : (long)blockIdx.x;
+    xy_tile<T, N, Op, NT>(op, g, partial, blk, lds, threadIdx.x);
+}
+
+template <typename T, int N, class Op, bool NT>
+__device__ __forceinline__ void xy_tile(const Op &op_, const XyGeom &g, double *partial, long blk, cpx<T> *lds,
+                                        int lane)
+{
+    constexpr int PPW = 64 / N;
+    constexpr int NC = Op::NC;
+    Op op = op_;
+    const int pl = lane / N, t = lane % N;
     const long plane = blk * PPW + pl;
     const int z = (int)(plane % g.nz);
     const int traj = (int)(plane / g.nz);
@@ -525,23 +550,38 @@ struct ZGeom {
 // blockIdx.y selects the first component (component-split launches of NC = 1).
 // HALF = true exchanges real and imaginary parts in two rounds through a tile
 // of NZ*PW reals (half the LDS, so more workgroups fit on a CU).
+template <typename T, bool HALF> using ZSlot = typename std::conditional<HALF, T, cpx<T>>::type;
+
+// One tile (PW pencils of one trajectory, component cb onward) of the z pass,
+// by one workgroup of (NZ / R1) * PW threads, exchanging through `lds`
+// (NZ * PW slots).  tid is threadIdx.x, passed as xy_tile's lane is.
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT, bool HALF>
+__device__ __forceinline__ void z_tile(const Op &op, const ZGeom &g, const cpx<T> *__restrict__ tw, long blk, int cb,
+                                       ZSlot<T, HALF> *lds, int tid);
+
 template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT = false,
           bool HALF = false>
 __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const cpx<T> *__restrict__ tw)
+{
+    __shared__ ZSlot<T, HALF> lds[NZ * PW];
+    const long blk = g.xcd ? xcd_remap(blockIdx.x, g.tiles_per_traj) : (long)blockIdx.x;
+    z_tile<T, NZ, R1, PW, SIGN_F, ROUNDTRIP, Op, NT, HALF>(op, g, tw, blk, blockIdx.y, lds, threadIdx.x);
+}
+
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT, bool HALF>
+__device__ __forceinline__ void z_tile(const Op &op_, const ZGeom &g, const cpx<T> *__restrict__ tw, long blk, int cb,
+                                       ZSlot<T, HALF> *lds, int tid)
 {
     constexpr int NW = NZ / R1;
     constexpr int KPT = R1 / NW;
     constexpr int NC = Op::NC;
     static_assert(R1 % NW == 0, "R1 must be a multiple of NW");
-    using Slot = typename std::conditional<HALF, T, cpx<T>>::type;
-    __shared__ Slot lds[NZ * PW];
-    const int tid = threadIdx.x;
+    using Slot = ZSlot<T, HALF>;
+    Op op = op_;
     const int p = tid % PW;
     // with PW = 64 a wavefront is one z subset: make w provably uniform so the
     // twiddle and kz-table loads that depend only on it become scalar loads
     const int w = PW == 64 ? __builtin_amdgcn_readfirstlane(tid / PW) : tid / PW;
-    const int cb = blockIdx.y;
-    const long blk = g.xcd ? xcd_remap(blockIdx.x, g.tiles_per_traj) : (long)blockIdx.x;
     const int traj = (int)(blk / g.tiles_per_traj);
     op.begin(traj, blk % g.tiles_per_traj == 0 && cb == 0);
     if (op.skip(traj)) return;

@@ -62,46 +62,76 @@ def load_traffic():
 
 
 def uneven_groups(B, psi_a, steps, autotune=True):
-    """256 trajectories at 128x16x16 (N = 30000) and 128 at 256x32x32 (N = 70000) stepped together
-    in one context (captured step blocks with a concurrent branch per group, autotuned schedules)
-    against the plain per-grid launch (one context per grid, one after the other, default
-    schedule, no graphs).  Same physics as the headline run; no sampling."""
-    ca = B.derive(B.default_model(**config1(ne=256)))
-    kb = config1(ne=128, N=70000, nvx=32, nvy=32, nvz=256)
+    """256 trajectories at 128x16x16 (N = 30000) and 128 at 256x32x32 (N = 70000), both at 20 us
+    steps (the 256x32x32 lattice is unstable at 40 us), no sampling, complex64.  Each launch form
+    is timed over the same steps:
+      plain_per_grid      one context per grid, one after the other, default schedules, no graphs
+      tuned_per_grid      one context per grid, one after the other, autotuned schedules, graphs
+      branches            one context, a concurrent graph branch per grid, autotuned schedules
+      one_launch          one context, one launch per pass over both grids' tiles (multi-group
+                          table, fixed schedules), graphs
+      one_launch_no_graph the same launch without graphs
+    The schedules are tuned once per grid and given to every form that uses them."""
+    ka = config1(ne=256, dt_evo=2e-5)
+    kb = config1(ne=128, N=70000, nvx=32, nvy=32, nvz=256, dt_evo=2e-5)
+    ca = B.derive(B.default_model(**ka))
     cb1 = B.derive(B.default_model(**dict(kb, ne=1)))
     with B.Solver(cb1, precision=B.FP64) as g:
         psi_b, rep_b, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
                                                    precision=B.FP64, max_iter=20000)
     cb = B.derive(B.default_model(**kb))
+    grids = (("128x16x16", ca, psi_a), ("256x32x32", cb, psi_b))
     ntot = ca.ne + cb.ne
 
     def timed(solver, n):
-        solver.propagate(50)
+        solver.propagate(64)
         solver.synchronize()
         t0 = time.perf_counter()
         solver.propagate(n)
         solver.synchronize()
         return time.perf_counter() - t0
 
-    s = B.Solver(ca, precision=B.FP32, groups=[ca, cb])
-    s.init_evolution(psi_a, SEED, group=0, equilibrate=False)
-    s.init_evolution(psi_b, SEED, group=1, equilibrate=False)
-    scheds = [s.autotune(steps=8, group=g)[0] if autotune else s.schedule(g) for g in (0, 1)]
-    el = timed(s, steps)
-    s.close()
-    plain = {}
-    for name, c, psi in (("128x16x16", ca, psi_a), ("256x32x32", cb, psi_b)):
+    def tps(sec):
+        return ntot * steps / sec
+
+    def one_grid(c, psi, sched, graphs):
         p = B.Solver(c, precision=B.FP32)
-        p.set_graphs(False)
+        p.set_graphs(graphs)
         p.init_evolution(psi, SEED, equilibrate=False)
-        plain[name] = {"trajectories": c.ne, "seconds": timed(p, steps)}
-        plain[name]["traj_steps_per_s"] = c.ne * steps / plain[name]["seconds"]
+        if sched is not None:
+            p.set_schedule(**sched)
+        sec = timed(p, steps)
         p.close()
-    t_plain = sum(v["seconds"] for v in plain.values())
-    return {"workload": "256 traj x 128x16x16 (N=30000) + 128 traj x 256x32x32 (N=70000), complex64, one context",
-            "steps": steps, "one_launch_traj_steps_per_s": ntot * steps / el,
-            "plain_per_grid_traj_steps_per_s": ntot * steps / t_plain, "speedup_over_plain": t_plain / el,
-            "plain_per_grid": plain, "schedules": {"128x16x16": scheds[0], "256x32x32": scheds[1]},
+        return sec
+
+    scheds = {}
+    for name, c, psi in grids:
+        p = B.Solver(c, precision=B.FP32)
+        p.init_evolution(psi, SEED, equilibrate=False)
+        best = p.autotune(steps=8)[0] if autotune else p.schedule()
+        scheds[name] = {k: v for k, v in best.items() if k != "ms_per_step"}
+        p.close()
+    forms = {}
+    sec = {n: one_grid(c, psi, None, False) for n, c, psi in grids}
+    forms["plain_per_grid"] = {"traj_steps_per_s": tps(sum(sec.values())), "seconds_per_grid": sec}
+    sec = {n: one_grid(c, psi, scheds[n], True) for n, c, psi in grids}
+    forms["tuned_per_grid"] = {"traj_steps_per_s": tps(sum(sec.values())), "seconds_per_grid": sec}
+    for form, multi, graphs in (("branches", False, True), ("one_launch", True, True),
+                                ("one_launch_no_graph", True, False)):
+        s = B.Solver(ca, precision=B.FP32, groups=[ca, cb])
+        s.init_evolution(psi_a, SEED, group=0, equilibrate=False)
+        s.init_evolution(psi_b, SEED, group=1, equilibrate=False)
+        if not multi:
+            for g, (n, _, _) in enumerate(grids):
+                s.set_schedule(group=g, **scheds[n])
+        s.set_graphs(graphs)
+        s.set_multi_launch(multi)
+        forms[form] = {"traj_steps_per_s": tps(timed(s, steps))}
+        s.close()
+    best_sep = max(forms[k]["traj_steps_per_s"] for k in ("plain_per_grid", "tuned_per_grid", "branches"))
+    return {"workload": "256 traj x 128x16x16 (N=30000) + 128 traj x 256x32x32 (N=70000), complex64, dt 20 us",
+            "steps": steps, "forms": forms, "schedules": scheds,
+            "one_launch_over_best_other": forms["one_launch"]["traj_steps_per_s"] / best_sep,
             "ground_state_256x32x32": {"iterations": rep_b["iterations"], "residual": rep_b["residual"]}}
 
 
@@ -312,6 +342,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-groups comparison")
     ap.add_argument("--uneven-steps", type=int, default=400)
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="run only the uneven-groups comparison and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): trajectories are independent and "
@@ -323,6 +355,14 @@ def main():
     if B.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if a.uneven_only:
+        c1 = B.derive(B.default_model(**config1(ne=1)))
+        with B.Solver(c1, precision=B.FP64) as g:
+            psi, _, _ = g.calculate_steady_state(method=B.GS_GRADIENT, rule=B.GS_RULE_RESIDUAL, tol=1e-6,
+                                                 precision=B.FP64, max_iter=5000)
+        print(json.dumps(uneven_groups(B, psi, a.uneven_steps, autotune=not a.no_autotune)))
+        return
 
     # ground state on the GPU: plain reference solve and residual-targeted solve
     c1 = B.derive(B.default_model(**config1(ne=1)))

@@ -429,6 +429,18 @@ int bec_calculate_averages(bec_ctx *ctx, int group, bec_avg_report *out);
  * inside bec_propagate (disabled: one launch per pass). */
 int bec_set_graphs(bec_ctx *ctx, int enable);
 
+/* Step every group of the context together: enable != 0 makes each pass of
+ * bec_propagate one launch over all groups' tiles (a table of tile ranges
+ * walked by the workgroups the GPU keeps resident), in place of one launch
+ * per group, or one graph branch per group.  Replaces the reference's
+ * per-frame, per-grid launches (calculateEvolution, wigner.cu:340-411).
+ * Needs FP32 states, at most 8 groups, and the same itmax == 1 and
+ * loss-noise setting in every group: BEC_ERR_UNSUPPORTED otherwise (the
+ * setting then stays off).  The launch uses fixed schedules (512-thread z
+ * workgroups, 64-thread xy workgroups), not the groups' own (bec_set_schedule).
+ * enable == 0 (the default) restores the per-group launches. */
+int bec_set_multi_launch(bec_ctx *ctx, int enable);
+
 /* Loss-induced Wigner noise during the evolution.  No counterpart in the
  * reference's code: its report derives the diffusion of the loss terms
  * (docs/wigner_simulation.report/report.tex:624-656; the poster
