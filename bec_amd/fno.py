@@ -45,6 +45,7 @@ class FnoConfig(C.Structure):
 
 EXPORTS = {
     "bec_gemm": (C.c_int, [C.POINTER(GemmDesc), _P]),
+    "bec_gemm_time": (C.c_int, [C.POINTER(GemmDesc), C.c_int, C.POINTER(C.c_double)]),
     "bec_fno_create": (C.c_int, [C.POINTER(FnoConfig), C.c_uint64, C.POINTER(_P)]),
     "bec_fno_destroy": (C.c_int, [_P]),
     "bec_fno_param_count": (C.c_longlong, [_P]),
@@ -119,7 +120,9 @@ def from_bf16_bits(b):
 
 
 def gemm(A, B, Cout, M, N, K, am, ak, bk, bn, cm, cn, batch=1, ab=None, bb=None, cb=None, alpha=1.0, beta=0.0,
-         bias=None, ksplit=1, work=None, stream=None):
+         bias=None, ksplit=1, work=None, stream=None, time_iters=0):
+    """C = alpha A B + beta C (+ bias) through bec_gemm; with time_iters > 0, bec_gemm_time
+    instead, returning the mean milliseconds of time_iters launches."""
     d = GemmDesc()
     d.A, d.B, d.C = A.ptr, B.ptr, Cout.ptr
     d.bias = bias.ptr if bias is not None else None
@@ -132,6 +135,12 @@ def gemm(A, B, Cout, M, N, K, am, ak, bk, bn, cm, cn, batch=1, ab=None, bb=None,
     d.ab = ab if ab is not None else m1(0)
     d.bb = bb if bb is not None else m1(0)
     d.cb = cb if cb is not None else m1(0)
+    if time_iters > 0:
+        ms = C.c_double()
+        st = flib().bec_gemm_time(C.byref(d), int(time_iters), C.byref(ms))
+        if st != OK:
+            raise BecError(st, "bec_gemm_time")
+        return ms.value
     st = flib().bec_gemm(C.byref(d), stream)
     if st != OK:
         raise BecError(st, "bec_gemm")

@@ -55,6 +55,11 @@ typedef struct bec_gemm_desc {
 
 int bec_gemm(const bec_gemm_desc *d, void *hip_stream);
 
+/* Event-timed bec_gemm on the null stream: one untimed launch, then iters
+ * launches; *ms_per_gemm is their mean duration.  The operands are the
+ * caller's (C is overwritten iters + 1 times). */
+int bec_gemm_time(const bec_gemm_desc *d, int iters, double *ms_per_gemm);
+
 /* ------------------------------------------------------------------ FNO */
 typedef struct bec_fno_config {
     int nz, ny, nx;       /* lattice (cells = nz ny nx) */

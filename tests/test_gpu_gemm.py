@@ -126,3 +126,40 @@ def test_split_k_is_deterministic(bec):
         F.gemm(dX, dX, dC, W, W, K, F.m1(1), F.m1(W), F.m1(W), F.m1(1), F.m1(W), F.m1(1), ksplit=16, work=work)
         outs.append(dC.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("a_bf16,b_bf16", [(False, False), (True, True), (False, True)])
+def test_eight_wide_loads_k_and_row_runs(bec, a_bf16, b_bf16):
+    """Sizes and strides that select the 8-wide operand loads: A row-major with
+    K a multiple of 8 (eight K values per load), B row-major with N a multiple of
+    8 (eight columns per load); M ragged, split K over a partial last tile."""
+    from bec_amd import fno as F
+    rng = np.random.default_rng(5)
+    M, N, K = 137, 72, 200
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    maps = (F.m1(K), F.m1(1), F.m1(N), F.m1(1), F.m1(N), F.m1(1))
+    err = _run(F, rng, A, B, (M, N), M, N, K, maps, a_bf16=a_bf16, b_bf16=b_bf16)
+    assert err < 1e-5, err
+    err = _run(F, rng, A, B, (M, N), M, N, K, maps, ksplit=3, a_bf16=a_bf16, b_bf16=b_bf16, beta=1.0)
+    assert err < 1e-5, err
+
+
+def test_eight_wide_loads_transposed_and_batched(bec):
+    """A column-major with M a multiple of 8 (eight rows per load), B column-major
+    with K a multiple of 8 (eight K values per load), batches 8-aligned, a
+    two-level K map on A whose runs are 8-aligned."""
+    from bec_amd import fno as F
+    rng = np.random.default_rng(6)
+    M, N, K, batch = 96, 50, 64, 3
+    A = rng.normal(size=(batch, K, M)).astype(np.float32)   # A[t][m][k] at t*K*M + k*M + m
+    B = rng.normal(size=(batch, N, K)).astype(np.float32)   # B[t][k][n] at t*N*K + n*K + k
+    maps = (F.m1(1), F.m1(M), F.m1(1), F.m1(K), F.m1(N), F.m1(1))
+    err = _run(F, rng, A, B, (batch, M, N), M, N, K, maps, batch=batch,
+               bmaps=(F.m1(K * M), F.m1(N * K), F.m1(M * N)), a_bf16=True)
+    assert err < 1e-5, err
+    # K as two runs of 32 (hi stride 40 * M keeps every run start 8-aligned)
+    A2 = rng.normal(size=(80 * M,)).astype(np.float32)
+    Bk = rng.normal(size=(K, N)).astype(np.float32)
+    err = _run(F, rng, A2, Bk, (M, N), M, N, K, (F.m1(1), F.m2(32, M, 40 * M), F.m1(N), F.m1(1), F.m1(N), F.m1(1)))
+    assert err < 1e-5, err
