@@ -158,6 +158,8 @@ EXPORTS = {
                                             C.POINTER(GsReport)]),
     "bec_ground_states_two_component": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, _P, C.POINTER(GsOptions),
                                                   _P, _P, C.POINTER(Gs2Report)]),
+    "bec_ground_states_two_component_net": (C.c_int, [C.POINTER(CalcParams), C.c_int, _P, _P, C.c_double,
+                                                      C.POINTER(GsOptions), _P, _P, C.POINTER(Gs2Report)]),
     "bec_state_residual": (C.c_int, [_P, C.c_int, _P, _D, _D]),
     "bec_run_2d": (C.c_int, [C.POINTER(Params2D), _P, _P, _P, _P, _P, C.POINTER(Report2D)]),
     "bec_fill_2d_params": (C.c_int, [C.POINTER(ModelParams), C.POINTER(Params2D)]),
@@ -286,9 +288,11 @@ def ground_states_batched(params, guesses=None, **opt):
     return out, [r.as_dict() for r in reps], st
 
 
-def ground_states_two_component(params, frac_a=None, guesses=None, **opt):
+def ground_states_two_component(params, frac_a=None, guesses=None, net=None, spin_weight=1.0, **opt):
     """Two-component ground states (coupled imaginary-time GPEs, gradient flow) for a
-    list of CalcParams sharing one lattice.  Returns (out [B][2][nz][ny][nx], reports, status)."""
+    list of CalcParams sharing one lattice.  Returns (out [B][2][nz][ny][nx], reports, status).
+    net: a device buffer (an object with .ptr) holding the surrogate network's output
+    [B][cells][2] for the start, converted on the GPU (bec_ground_states_two_component_net)."""
     L = lib()
     B = len(params)
     arr = (CalcParams * B)(*params)
@@ -304,8 +308,14 @@ def ground_states_two_component(params, frac_a=None, guesses=None, **opt):
     g = None if guesses is None else _c128(guesses)
     if g is not None and g.size != out.size:
         raise ValueError("guesses must hold batch * 2 * cells values")
-    st = L.bec_ground_states_two_component(arr, B, None if fr is None else _ptr(fr), None if g is None else _ptr(g),
-                                           C.byref(o), None, _ptr(out), reps)
+    if net is not None:
+        if g is not None:
+            raise ValueError("give guesses or net, not both")
+        st = L.bec_ground_states_two_component_net(arr, B, None if fr is None else _ptr(fr), net.ptr,
+                                                   float(spin_weight), C.byref(o), None, _ptr(out), reps)
+    else:
+        st = L.bec_ground_states_two_component(arr, B, None if fr is None else _ptr(fr),
+                                               None if g is None else _ptr(g), C.byref(o), None, _ptr(out), reps)
     if st not in (OK, ERR_NOT_CONVERGED):
         check(st, "bec_ground_states_two_component")
     return out, [r.as_dict() for r in reps], st

@@ -334,3 +334,8 @@ extern "C" int bec_gemm_time(const bec_gemm_desc *d, int iters, double *ms_per_g
     *ms_per_gemm = (double)ms / iters;
     return BEC_OK;
 }
+
+extern "C" int bec_device_synchronize(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? BEC_OK : BEC_ERR_HIP;
+}

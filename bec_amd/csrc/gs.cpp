@@ -316,8 +316,11 @@ int gs_validate(const bec_calc_params *params, int batch, const bec_gs_options *
     return BEC_OK;
 }
 
+// net != nullptr (two-component solves only): the start is the surrogate
+// network's output on the device, net [batch][cells][2], with its spin weight
 int gs_solve(const bec_calc_params *params, int batch, const double *guesses, const bec_gs_options *opt,
-             hipStream_t s, double *out_x, bec_gs_report *reps, const double *frac_a, bec_gs2_report *reps2)
+             hipStream_t s, double *out_x, bec_gs_report *reps, const double *frac_a, bec_gs2_report *reps2,
+             const float *net = nullptr, double spin_weight = 1.0)
 {
     int vst = gs_validate(params, batch, opt);
     if (vst != BEC_OK) return vst;
@@ -329,8 +332,20 @@ int gs_solve(const bec_calc_params *params, int batch, const double *guesses, co
     const int ntraj = d.ntraj;
     const long n = w.g.cells * ntraj;
 
-    // start: guesses or Thomas-Fermi (fillWithTFGroundState), normalised to N
-    if (guesses) {
+    // start: guesses, the network's output, or Thomas-Fermi (fillWithTFGroundState), normalised to N
+    if (net) {
+        // the amplitude unit sqrt(mu / g11) of each set, as the host conversion uses it
+        std::vector<double> unit(batch);
+        for (int j = 0; j < batch; ++j) unit[j] = std::sqrt(params[j].mu / params[j].g11);
+        double *d_unit = nullptr;
+        GS_HIP(dev_alloc((void **)&d_unit, sizeof(double) * batch));
+        hipError_t e = hipMemcpyAsync(d_unit, unit.data(), sizeof(double) * batch, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = gs_net_start(d, w.X, net, d_unit, spin_weight, 1.0 / std::sqrt(2.0), s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        void *p = d_unit;
+        dev_free(p);
+        GS_HIP(e);
+    } else if (guesses) {
         st = upload_complex(guesses, w.X, n, o.precision, s);
         if (st != BEC_OK) return st;
     } else {
@@ -497,9 +512,29 @@ extern "C" int bec_ground_states_batched(const bec_calc_params *params, int batc
     return st;
 }
 
+static int two_component(const bec_calc_params *params, int batch, const double *frac_a, const double *guesses,
+                         const float *net, double spin_weight, const bec_gs_options *opt, void *stream,
+                         double *out_x, bec_gs2_report *reps);
+
 extern "C" int bec_ground_states_two_component(const bec_calc_params *params, int batch, const double *frac_a,
                                                const double *guesses, const bec_gs_options *opt, void *stream,
                                                double *out_x, bec_gs2_report *reps)
+{
+    return two_component(params, batch, frac_a, guesses, nullptr, 1.0, opt, stream, out_x, reps);
+}
+
+extern "C" int bec_ground_states_two_component_net(const bec_calc_params *params, int batch, const double *frac_a,
+                                                   const float *d_net, double spin_weight,
+                                                   const bec_gs_options *opt, void *stream, double *out_x,
+                                                   bec_gs2_report *reps)
+{
+    if (!d_net || !(spin_weight > 0.0)) return BEC_ERR_INVALID;
+    return two_component(params, batch, frac_a, nullptr, d_net, spin_weight, opt, stream, out_x, reps);
+}
+
+static int two_component(const bec_calc_params *params, int batch, const double *frac_a, const double *guesses,
+                         const float *net, double spin_weight, const bec_gs_options *opt, void *stream,
+                         double *out_x, bec_gs2_report *reps)
 {
     bec_gs_options o;
     if (opt) o = *opt;
@@ -528,8 +563,20 @@ extern "C" int bec_ground_states_two_component(const bec_calc_params *params, in
     if (!s) {
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BEC_ERR_HIP;
         own = true;
+        if (net) {
+            // the network's output was written on the null stream: order after it
+            hipEvent_t ev = nullptr;
+            hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventRecord(ev, nullptr);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+            if (ev) (void)hipEventDestroy(ev);
+            if (e != hipSuccess) {
+                (void)hipStreamDestroy(s);
+                return BEC_ERR_HIP;
+            }
+        }
     }
-    int st = bec::gs_solve(params, batch, guesses, &o, s, out_x, nullptr, fr.data(), reps);
+    int st = bec::gs_solve(params, batch, guesses, &o, s, out_x, nullptr, fr.data(), reps, net, spin_weight);
     if (own) (void)hipStreamDestroy(s);
     return st;
 }

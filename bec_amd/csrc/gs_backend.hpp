@@ -76,5 +76,11 @@ hipError_t gs_norm_only(const DevGroup &g, const void *psi, const GsScalars &sc,
 hipError_t gs_apply_scale(const DevGroup &g, void *psi, const GsScalars &sc, hipStream_t s);
 // Thomas-Fermi start (fillWithTFGroundState, wigner_kernel.cu:116-125), per record mu
 hipError_t gs_tf_start(const DevGroup &g, void *psi, const double *mu_tf, hipStream_t s);
+// Two-component start from the surrogate network's output net [sets][cells][2]
+// (channels s and w d): a = ((s + d) r) u_j, b = ((s - d) r) u_j with
+// d = net1 / w and r = 1/sqrt(2), real, into trajectories 2j and 2j + 1 (FP64
+// arithmetic in the order of the host conversion, so both give the same bits)
+hipError_t gs_net_start(const DevGroup &g, void *psi, const float *net, const double *unit, double spin_weight,
+                        double r, hipStream_t s);
 
 } // namespace bec

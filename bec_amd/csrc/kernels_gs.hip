@@ -672,6 +672,35 @@ hipError_t gs_apply_scale(const DevGroup &g, void *psi, const GsScalars &sc, hip
     return hipGetLastError();
 }
 
+template <typename T>
+__global__ void net_start_kernel(const float *__restrict__ net, cpx<T> *__restrict__ psi, long cells, long n,
+                                 const double *__restrict__ unit, double w, double r)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;   // i = set * cells + cell
+    if (i >= n) return;
+    const long j = i / cells, cell = i - j * cells;
+    const double sv = (double)net[2 * i], dv = (double)net[2 * i + 1] / w;
+    const double u = unit[j];
+    const double a = ((sv + dv) * r) * u, b = ((sv - dv) * r) * u;
+    psi[(2 * j) * cells + cell] = mk<T>((T)a, (T)0);
+    psi[(2 * j + 1) * cells + cell] = mk<T>((T)b, (T)0);
+}
+
+hipError_t gs_net_start(const DevGroup &g, void *psi, const float *net, const double *unit, double spin_weight,
+                        double r, hipStream_t s)
+{
+    const long cells = (long)g.nx * g.ny * g.nz;
+    const long n = cells * (g.ntraj / 2);
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (g.prec == 0)
+        hipLaunchKernelGGL(net_start_kernel<float>, grid, block, 0, s, net, (cpx<float> *)psi, cells, n, unit,
+                           spin_weight, r);
+    else
+        hipLaunchKernelGGL(net_start_kernel<double>, grid, block, 0, s, net, (cpx<double> *)psi, cells, n, unit,
+                           spin_weight, r);
+    return hipGetLastError();
+}
+
 hipError_t gs_tf_start(const DevGroup &g, void *psi, const double *mu_tf, hipStream_t s)
 {
     const long n = (long)g.nx * g.ny * g.nz * g.ntraj;

@@ -568,23 +568,58 @@ __global__ void __launch_bounds__((NZ / R1) * PW) z_kernel(Op op, ZGeom g, const
     z_tile<T, NZ, R1, PW, SIGN_F, ROUNDTRIP, Op, NT, HALF>(op, g, tw, blk, blockIdx.y, lds, threadIdx.x);
 }
 
+// The loads of one z tile: thread (p, w) reads z = w + NW j of pencil q, j < R1.
+template <typename T, int NZ, int R1, int PW, class Op, bool NT>
+__device__ __forceinline__ void z_tile_load(const Op &op, const ZGeom &g, long blk, int cb, int tid,
+                                            cpx<T> (&v)[Op::NC][R1])
+{
+    constexpr int NW = NZ / R1;
+    const int p = tid % PW;
+    const int w = PW == 64 ? __builtin_amdgcn_readfirstlane(tid / PW) : tid / PW;
+    const int traj = (int)(blk / g.tiles_per_traj);
+    const int q = (int)(blk % g.tiles_per_traj) * PW + p;
+    const long base = (long)traj * NZ * g.nxy + q;
+#pragma unroll
+    for (int c = 0; c < Op::NC; ++c) {
+        const cpx<T> *s = op.src(c + cb);
+#pragma unroll
+        for (int j = 0; j < R1; ++j) v[c][j] = ld<NT, T>(s + base + (long)(w + NW * j) * g.nxy);
+    }
+}
+
+// The rest of one z tile (transforms, operator, stores) on the values
+// z_tile_load read, with op already begun for the tile's trajectory.
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT, bool HALF>
+__device__ __forceinline__ void z_tile_compute(Op &op, const ZGeom &g, const cpx<T> *__restrict__ tw, long blk,
+                                               int cb, ZSlot<T, HALF> *lds, int tid, cpx<T> (&v)[Op::NC][R1]);
+
 template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT, bool HALF>
 __device__ __forceinline__ void z_tile(const Op &op_, const ZGeom &g, const cpx<T> *__restrict__ tw, long blk, int cb,
                                        ZSlot<T, HALF> *lds, int tid)
+{
+    Op op = op_;
+    const int traj = (int)(blk / g.tiles_per_traj);
+    op.begin(traj, blk % g.tiles_per_traj == 0 && cb == 0);
+    if (op.skip(traj)) return;
+    cpx<T> v[Op::NC][R1];
+    z_tile_load<T, NZ, R1, PW, Op, NT>(op, g, blk, cb, tid, v);
+    z_tile_compute<T, NZ, R1, PW, SIGN_F, ROUNDTRIP, Op, NT, HALF>(op, g, tw, blk, cb, lds, tid, v);
+}
+
+template <typename T, int NZ, int R1, int PW, int SIGN_F, bool ROUNDTRIP, class Op, bool NT, bool HALF>
+__device__ __forceinline__ void z_tile_compute(Op &op, const ZGeom &g, const cpx<T> *__restrict__ tw, long blk,
+                                               int cb, ZSlot<T, HALF> *lds, int tid, cpx<T> (&v)[Op::NC][R1])
 {
     constexpr int NW = NZ / R1;
     constexpr int KPT = R1 / NW;
     constexpr int NC = Op::NC;
     static_assert(R1 % NW == 0, "R1 must be a multiple of NW");
     using Slot = ZSlot<T, HALF>;
-    Op op = op_;
     const int p = tid % PW;
     // with PW = 64 a wavefront is one z subset: make w provably uniform so the
     // twiddle and kz-table loads that depend only on it become scalar loads
     const int w = PW == 64 ? __builtin_amdgcn_readfirstlane(tid / PW) : tid / PW;
     const int traj = (int)(blk / g.tiles_per_traj);
-    op.begin(traj, blk % g.tiles_per_traj == 0 && cb == 0);
-    if (op.skip(traj)) return;
     const int q = (int)(blk % g.tiles_per_traj) * PW + p;
     const long base = (long)traj * NZ * g.nxy + q;
     const int kk0 = w * KPT;
@@ -645,14 +680,6 @@ __device__ __forceinline__ void z_tile(const Op &op_, const ZGeom &g, const cpx<
             __syncthreads();
         }
     };
-
-    cpx<T> v[NC][R1];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        const cpx<T> *s = op.src(c + cb);
-#pragma unroll
-        for (int j = 0; j < R1; ++j) v[c][j] = ld<NT, T>(s + base + (long)(w + NW * j) * g.nxy);
-    }
 
 #pragma unroll
     for (int c = 0; c < NC; ++c) {

@@ -46,6 +46,7 @@ class FnoConfig(C.Structure):
 EXPORTS = {
     "bec_gemm": (C.c_int, [C.POINTER(GemmDesc), _P]),
     "bec_gemm_time": (C.c_int, [C.POINTER(GemmDesc), C.c_int, C.POINTER(C.c_double)]),
+    "bec_device_synchronize": (C.c_int, []),
     "bec_fno_create": (C.c_int, [C.POINTER(FnoConfig), C.c_uint64, C.POINTER(_P)]),
     "bec_fno_destroy": (C.c_int, [_P]),
     "bec_fno_param_count": (C.c_longlong, [_P]),
@@ -209,6 +210,13 @@ class FNO:
         dy = DeviceArray((b, self.cells, self.cfg.c_out), np.float32)
         self.forward(dx, dy, b)
         return dy.download()
+
+
+def device_synchronize():
+    """Wait for the device (bec_device_synchronize)."""
+    st = flib().bec_device_synchronize()
+    if st != OK:
+        raise BecError(st, "bec_device_synchronize")
 
 
 def gs2_features(calc, scaled, d_x, stream=None):

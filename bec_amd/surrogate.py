@@ -127,10 +127,12 @@ def guess_error(pred, target):
     return float(np.sqrt((np.sum((pa - ta) ** 2) + np.sum((pb - tb) ** 2)) / (np.sum(ta ** 2) + np.sum(tb ** 2))))
 
 
-def solve(calcs, guess=None, tol=1e-6, max_iter=20000, **opts):
-    """Two-component ground states, FP64; opts (tau, beta, alpha) override the coupled flow's defaults."""
-    out, reps, st = B.ground_states_two_component(calcs, guesses=guess, tol=tol, max_iter=max_iter,
-                                                  precision=B.FP64, **opts)
+def solve(calcs, guess=None, tol=1e-6, max_iter=20000, net=None, **opts):
+    """Two-component ground states, FP64; opts (tau, beta, alpha) override the coupled flow's defaults.
+    guess: host guesses [B][2][cells]; net: the network's output left on the device
+    ([B][cells][C_OUT]), converted to the start on the GPU."""
+    out, reps, st = B.ground_states_two_component(calcs, guesses=guess, net=net, spin_weight=SPIN_WEIGHT, tol=tol,
+                                                  max_iter=max_iter, precision=B.FP64, **opts)
     return out, reps, st
 
 
@@ -216,11 +218,13 @@ def flow(spec):
     return dict(tau=tau, beta=beta)
 
 
-def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None):
+def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None, device_start=True):
     """Per held-out set: solve from TF and from the network guess (one set per solve).
-    inference_ms is the host wall clock of the whole guess: input features
-    (built on the device), forward pass, download and conversion to the
-    solver's layout.
+    inference_ms is the host wall clock of the guess: input features (built on
+    the device) and forward pass, then either (device_start) the solver reads the
+    network's output on the device and converts it in a kernel before its timed
+    iterations, as it fills the Thomas-Fermi start, or the output is downloaded
+    and converted to the solver's layout on the host (included in inference_ms).
     tf_opts / nn_opts: the flow's step and momentum for each start."""
     rows = []
     cells = X.shape[1]
@@ -232,10 +236,16 @@ def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None)
         t0 = time.perf_counter()
         F.gs2_features(c, scaled(t), dx)
         net.forward(dx, dy, 1)
-        pred = dy.download()[0]
-        g = guesses(c, pred)[None]
-        t_inf = (time.perf_counter() - t0) * 1e3
-        _, r_nn, st_nn = solve([c], guess=g, tol=tol, **(nn_opts or {}))
+        if device_start:
+            F.device_synchronize()
+            t_inf = (time.perf_counter() - t0) * 1e3
+            _, r_nn, st_nn = solve([c], net=dy, tol=tol, **(nn_opts or {}))
+            pred = dy.download()[0]
+        else:
+            pred = dy.download()[0]
+            g = guesses(c, pred)[None]
+            t_inf = (time.perf_counter() - t0) * 1e3
+            _, r_nn, st_nn = solve([c], guess=g, tol=tol, **(nn_opts or {}))
         rows.append({"theta": [float(v) for v in t], "tf_iters": r_tf[0]["iterations"],
                      "tf_ms": r_tf[0]["elapsed_ms"], "tf_residual": r_tf[0]["residual"],
                      "nn_iters": r_nn[0]["iterations"], "nn_ms": r_nn[0]["elapsed_ms"],
@@ -246,11 +256,12 @@ def evaluate(net, thetas, X, Y, tol=1e-6, log=print, tf_opts=None, nn_opts=None)
     return rows
 
 
-def evaluate_batched(net, thetas, tol=1e-6, tf_opts=None, nn_opts=None):
+def evaluate_batched(net, thetas, tol=1e-6, tf_opts=None, nn_opts=None, device_start=True):
     """All sets in one batched solve from each start (host wall clock).  The
     network-started time includes the guesses, made in batches of the
-    network's max_batch (features on the device, forward pass, download,
-    conversion)."""
+    network's max_batch (features on the device, forward pass) and left on the
+    device for the solver's start kernel (device_start), or downloaded and
+    converted on the host."""
     calcs = [calc_for(t) for t in thetas]
     n, cells = len(calcs), calcs[0].cells
     solve(calcs[:2], tol=tol, **(tf_opts or {}))     # first-call set-up outside the timing
@@ -259,19 +270,24 @@ def evaluate_batched(net, thetas, tol=1e-6, tf_opts=None, nn_opts=None):
     t_tf = (time.perf_counter() - t0) * 1e3
     b = net.cfg.max_batch
     dx = F.DeviceArray((b, cells, C_IN), np.float32)
-    dy = F.DeviceArray((b, cells, C_OUT), np.float32)
-    g = np.empty((n, 2, cells), dtype=np.complex128)
+    dy = F.DeviceArray((n if device_start else b, cells, C_OUT), np.float32)
+    g = None if device_start else np.empty((n, 2, cells), dtype=np.complex128)
     t0 = time.perf_counter()
     for s0 in range(0, n, b):
         m = min(b, n - s0)
         for j in range(m):
             F.gs2_features(calcs[s0 + j], scaled(thetas[s0 + j]), F.DeviceView(dx, j * cells * C_IN * 4))
-        net.forward(dx, dy, m)
-        pred = dy.download()
-        for j in range(m):
-            g[s0 + j] = guesses(calcs[s0 + j], pred[j])
+        if device_start:
+            net.forward(dx, F.DeviceView(dy, s0 * cells * C_OUT * 4), m)
+        else:
+            net.forward(dx, dy, m)
+            pred = dy.download()
+            for j in range(m):
+                g[s0 + j] = guesses(calcs[s0 + j], pred[j])
+    if device_start:
+        F.device_synchronize()
     t_inf = (time.perf_counter() - t0) * 1e3
-    _, r_nn, st_nn = solve(calcs, guess=g, tol=tol, **(nn_opts or {}))
+    _, r_nn, st_nn = solve(calcs, guess=g, net=dy if device_start else None, tol=tol, **(nn_opts or {}))
     t_nn = (time.perf_counter() - t0) * 1e3
     dx.free()
     dy.free()

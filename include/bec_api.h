@@ -180,6 +180,19 @@ int bec_ground_states_two_component(const bec_calc_params *params, int batch, co
                                     const double *guesses, const bec_gs_options *opt, void *hip_stream,
                                     double *out_x, bec_gs2_report *reps);
 
+/* The same solve started from the learned start's network output where the
+ * network left it, in device memory (no host round trip; no counterpart in the
+ * reference).  d_net holds batch x cells x 2 floats, the network's output
+ * channels s and spin_weight * d of each set; the start is
+ * a = ((s + d)/sqrt2) u, b = ((s - d)/sqrt2) u with u = sqrt(mu / g11) of the
+ * set, in FP64 in that order (the same bits as building the guesses on the host
+ * and passing them to bec_ground_states_two_component).  d_net must be
+ * complete on hip_stream when it is given, or on the null stream when
+ * hip_stream is NULL. */
+int bec_ground_states_two_component_net(const bec_calc_params *params, int batch, const double *frac_a,
+                                        const float *d_net, double spin_weight, const bec_gs_options *opt,
+                                        void *hip_stream, double *out_x, bec_gs2_report *reps);
+
 /* Residual ||H psi - mu psi||/(|mu| ||psi||) and mu of an x-space state, FP64 on the GPU. */
 int bec_state_residual(bec_ctx *ctx, int group, const double *psi_x, double *residual, double *mu);
 

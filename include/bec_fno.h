@@ -60,6 +60,10 @@ int bec_gemm(const bec_gemm_desc *d, void *hip_stream);
  * caller's (C is overwritten iters + 1 times). */
 int bec_gemm_time(const bec_gemm_desc *d, int iters, double *ms_per_gemm);
 
+/* Waits for all device work of the calling process (the network's kernels
+ * run on the null stream unless a stream is given). */
+int bec_device_synchronize(void);
+
 /* ------------------------------------------------------------------ FNO */
 typedef struct bec_fno_config {
     int nz, ny, nx;       /* lattice (cells = nz ny nx) */

@@ -135,3 +135,29 @@ def test_network_started_state_evolves_like_thomas_fermi_started(bec, oracle, tr
     assert np.max(np.abs(v_nn - v_tf)) < 1e-4
     assert np.max(np.abs(v_nn - v_or)) < 1e-4
     assert v_or.min() < 0.999   # the visibility moves over the run: the comparison is not of constants
+
+
+def test_device_start_equals_host_start(bec, trained):
+    """bec_ground_states_two_component_net converts the network's output to the
+    solver's start on the GPU: the same bits as the host conversion passed as
+    guesses, so the same iterations and the same states, for a batch of sets."""
+    from bec_amd import fno as F
+    from bec_amd import surrogate as S
+    net, thetas, X, Y, _ = trained
+    ths = thetas[64:67]
+    calcs = [S.calc_for(t) for t in ths]
+    cells = calcs[0].cells
+    dx = F.DeviceArray((len(ths), cells, S.C_IN), np.float32)
+    dy = F.DeviceArray((len(ths), cells, S.C_OUT), np.float32)
+    for j, t in enumerate(ths):
+        F.gs2_features(calcs[j], S.scaled(t), F.DeviceView(dx, j * cells * S.C_IN * 4))
+    net.forward(dx, dy, len(ths))
+    pred = dy.download()
+    g = np.stack([S.guesses(c, p) for c, p in zip(calcs, pred)])
+    host, r_host, st_host = S.solve(calcs, guess=g, **S.flow(S.NN_FLOW))
+    dev, r_dev, st_dev = S.solve(calcs, net=dy, **S.flow(S.NN_FLOW))
+    dx.free()
+    dy.free()
+    assert st_host == 0 and st_dev == 0
+    assert [r["iterations"] for r in r_host] == [r["iterations"] for r in r_dev]
+    assert np.array_equal(host, dev)
