@@ -147,7 +147,9 @@ __device__ __forceinline__ void xy_tile(const Op &op_, const XyGeom &g, double *
     const int pl = lane / N, t = lane % N;
     const long plane = blk * PPW + pl;
     const int z = (int)(plane % g.nz);
-    const int traj = (int)(plane / g.nz);
+    // PPW divides nz, so all planes of the tile belong to one trajectory; taken
+    // from blk alone it is uniform, and its parameters load into scalar registers
+    const int traj = (int)((blk * PPW) / g.nz);
     op.begin(traj, blk % (g.nz / PPW) == 0);
     if (op.skip(traj)) return;   // uniform: a workgroup never spans two trajectories
     cpx<T> *tile = lds + pl * N * (N + 1);
