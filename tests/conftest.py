@@ -7,6 +7,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# The oracle's loops are OpenMP-parallel over every core.  Under pytest-xdist each
+# worker would start one such pool, and the pools' spinning threads oversubscribe
+# the cores by the worker count (a test that takes seconds alone then takes minutes),
+# so each worker gets its share of the cores unless OMP_NUM_THREADS is already set.
+_workers = os.environ.get("PYTEST_XDIST_WORKER_COUNT")
+if _workers and "OMP_NUM_THREADS" not in os.environ:
+    os.environ["OMP_NUM_THREADS"] = str(max(1, (os.cpu_count() or 1) // max(1, int(_workers))))
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
