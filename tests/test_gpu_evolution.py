@@ -51,6 +51,36 @@ def test_propagate_matches_oracle_per_trajectory(bec, oracle, gs, prec, tol):
     assert eb.max() < tol, eb
 
 
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 2e-5)])
+def test_strong_losses_take_both_loss_factor_forms(bec, oracle, gs, prec, tol):
+    """Loss rates 100x config 1's: the per-step loss exponent y = -dt (l111 na^2 + l12 nb)/2 of
+    component a then straddles 2^-8, where the FP32 step switches from its cubic expm1 to
+    exp(y) - 1, so both forms run in the same waves; each trajectory must still follow the
+    FP64 oracle."""
+    loss = dict(gamma111=5.4e-28, gamma12=0.78e-11, gamma22=1.5e-12)
+    c, oc = _pair(bec, oracle, prec, ne=3, **loss)
+    oa, ob = oracle.init_evolution(oc, gs, SEED, equilibrate=False)
+    na = np.abs(oracle.to_xspace(oc, oa)) ** 2
+    nb = np.abs(oracle.to_xspace(oc, ob)) ** 2
+    ya = -0.5 * oc.dt_evo * (oc.l111 * na ** 2 + oc.l12 * nb)
+    big = np.mean(np.abs(ya) >= 2.0 ** -8)
+    assert 0.001 < big < 0.5, big
+    nsteps = 25
+    with bec.Solver(c, precision=prec) as s:
+        s.init_evolution(gs, SEED, equilibrate=False)
+        s.propagate(nsteps)
+        ax, bx = s.get_state(bec.SPACE_X)
+    oa, ob = oracle.propagate(oc, oa, ob, oc.dt_evo, nsteps)
+    ea = rel_l2(ax, oracle.to_xspace(oc, oa), axis=(1, 2, 3))
+    eb = rel_l2(bx, oracle.to_xspace(oc, ob), axis=(1, 2, 3))
+    assert ea.max() < tol, ea
+    assert eb.max() < tol, eb
+    # the losses act: the atoms of component a fall by more than a percent over the run
+    n0 = np.sum(na, axis=(1, 2, 3))
+    n1 = np.sum(np.abs(ax) ** 2, axis=(1, 2, 3))
+    assert np.all(n1 < 0.99 * n0), n1 / n0
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_samples_and_observables(bec, oracle, gs, prec):
     c, oc = _pair(bec, oracle, prec, ne=4)
