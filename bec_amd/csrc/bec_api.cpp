@@ -125,6 +125,54 @@ TrajParams make_traj_params(const bec_calc_params &c, double dt)
     return p;
 }
 
+// (cos a, sin a) in FP32 as the float pair within two ulps of each whose
+// modulus error c^2 + s^2 - 1 is smallest (ties: the smaller angle error), the
+// rule of the in-register twiddles (fft_reg.hpp).  Rounded separately to the
+// nearest floats, a table entry keeps a modulus error of up to 6e-8 that every
+// step applies again to the same mode, a systematic drift of the FP32 norm.
+static void unit_f32(double a, float *cr, float *sr)
+{
+    const double c0 = cos(a), s0 = sin(a);
+    float cs[5], ss[5];
+    cs[2] = (float)c0;
+    ss[2] = (float)s0;
+    for (int i = 1; i <= 2; ++i) {
+        cs[2 + i] = nextafterf(cs[1 + i], INFINITY);
+        cs[2 - i] = nextafterf(cs[3 - i], -INFINITY);
+        ss[2 + i] = nextafterf(ss[1 + i], INFINITY);
+        ss[2 - i] = nextafterf(ss[3 - i], -INFINITY);
+    }
+    double be = 1e300, bp = 1e300;
+    for (float c : cs)
+        for (float s : ss) {
+            const double e = fabs((double)c * c + (double)s * s - 1.0);
+            const double pe = fabs(c0 * (double)s - s0 * (double)c);
+            if (e < be || (e == be && pe < bp)) {
+                be = e;
+                bp = pe;
+                *cr = c;
+                *sr = s;
+            }
+        }
+}
+
+// exp(i a) * scale into entry idx: FP32 from unit_f32 (scale a power of two,
+// so the scaling is exact), FP64 from cos and sin directly
+static void put_unit(std::vector<char> &buf, long idx, int prec, double a, double scale)
+{
+    if (prec == 0) {
+        float c, s;
+        unit_f32(a, &c, &s);
+        float *f = (float *)buf.data();
+        f[2 * idx] = (float)(c * scale);
+        f[2 * idx + 1] = (float)(s * scale);
+    } else {
+        double *d = (double *)buf.data();
+        d[2 * idx] = cos(a) * scale;
+        d[2 * idx + 1] = sin(a) * scale;
+    }
+}
+
 static void put_complex(std::vector<char> &buf, long idx, int prec, double re, double im)
 {
     if (prec == 0) {
@@ -161,7 +209,7 @@ int group_build_tables(Group &g, double dt, hipStream_t s)
                         put_complex(hxy, idx, prec, exp(-c.dt_steady / 2.0 * e), 0.0);
                     } else {
                         double ph = e * dt * (v == KT_RT_HALF ? 0.5 : 1.0);
-                        put_complex(hxy, idx, prec, cos(ph), sin(ph));
+                        put_unit(hxy, idx, prec, ph, 1.0);
                     }
                 }
             for (int z = 0; z < nz; ++z) {
@@ -172,7 +220,7 @@ int group_build_tables(Group &g, double dt, hipStream_t s)
                     put_complex(hz, idx, prec, exp(-c.dt_steady / 2.0 * e) / nz, 0.0);
                 } else {
                     double ph = e * dt * (v == KT_RT_HALF ? 0.5 : 1.0);
-                    put_complex(hz, idx, prec, cos(ph) / nz, sin(ph) / nz);
+                    put_unit(hz, idx, prec, ph, 1.0 / nz);   // nz is a power of two
                 }
             }
         }
@@ -226,7 +274,7 @@ int group_setup(Group &g, const bec_calc_params &c, int prec, int ntraj, bool al
     std::vector<char> tw(es * c.nvz);
     for (int j = 0; j < c.nvz; ++j) {
         double ang = 2.0 * M_PI * j / c.nvz;
-        put_complex(tw, j, prec, cos(ang), sin(ang));
+        put_unit(tw, j, prec, ang, 1.0);
     }
     BEC_HIP(hipMemcpy(d.tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
     g.partial_slots = xy_workgroups(d, ntraj);
@@ -1154,16 +1202,7 @@ extern "C" int bec_fft_plan3d(bec_fft_plan **plan, int nz, int ny, int nx, int p
     p->dev.ntraj = batch;
     const size_t es = elem_size(precision);
     std::vector<char> tw(es * nz);
-    for (int j = 0; j < nz; ++j) {
-        double ang = 2.0 * M_PI * j / nz;
-        if (precision == 0) {
-            ((float *)tw.data())[2 * j] = (float)cos(ang);
-            ((float *)tw.data())[2 * j + 1] = (float)sin(ang);
-        } else {
-            ((double *)tw.data())[2 * j] = cos(ang);
-            ((double *)tw.data())[2 * j + 1] = sin(ang);
-        }
-    }
+    for (int j = 0; j < nz; ++j) put_unit(tw, j, precision, 2.0 * M_PI * j / nz, 1.0);
     if (dev_alloc(&p->dev.tw, tw.size()) != hipSuccess ||
         hipMemcpy(p->dev.tw, tw.data(), tw.size(), hipMemcpyHostToDevice) != hipSuccess) {
         dev_free(p->dev.tw);

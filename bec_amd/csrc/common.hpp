@@ -23,6 +23,14 @@ template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
 {
     return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// a * b with its fused multiply-adds written out, so that every kernel that
+// applies it rounds alike (which product of cmul the compiler fuses can differ
+// between two kernels that inline it)
+template <typename T> __device__ __forceinline__ cpx<T> cmul_fixed(cpx<T> a, cpx<T> b)
+{
+    if constexpr (sizeof(T) == 4) return mk<T>(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+    else return mk<T>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
 // conj(a) * b
 template <typename T> __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b)
 {

@@ -692,7 +692,8 @@ __device__ __forceinline__ void z_tile_compute(Op &op, const ZGeom &g, const cpx
 
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        dft<T, R1, SIGN_F>(v[c]);
+        // the first stage's twiddles also offset the modulus errors of the NW-point stage
+        dft<T, R1, SIGN_F, NW>(v[c]);
 #pragma unroll
         for (int k = 1; k < R1; ++k) {
             cpx<T> f = tw[(w * k) % NZ];
@@ -718,7 +719,7 @@ __device__ __forceinline__ void z_tile_compute(Op &op, const ZGeom &g, const cpx
                 }
             }
             from_k(X, v[c]);
-            dft<T, R1, -SIGN_F>(v[c]);
+            dft<T, R1, -SIGN_F, NW>(v[c]);
         } else {
             cpx<T> *d = op.dst(c + cb);
 #pragma unroll
@@ -757,13 +758,15 @@ template <typename T, int NZ, int R1, int NC_> struct OpKinetic : NoSkip {
         const int rec = traj >> rec_shift;
         const cpx<T> fxy = kxy[(long)rec * nxy + q];
         const cpx<T> *kzr = kz + (long)rec * NZ;
+        // the two factors one after the other (the same instruction count as
+        // forming their product first): the modulus error a mode sees every step
+        // is then that of the two tables' entries, chosen near one, rather than
+        // the rounding of their product
 #pragma unroll
         for (int i = 0; i < KPT; ++i)
 #pragma unroll
-            for (int m = 0; m < NW; ++m) {
-                const cpx<T> f = cmul<T>(fxy, kzr[kk0 + i + R1 * m]);
-                X[i][m] = cmul<T>(X[i][m], f);
-            }
+            for (int m = 0; m < NW; ++m)
+                X[i][m] = cmul_fixed<T>(cmul_fixed<T>(X[i][m], fxy), kzr[kk0 + i + R1 * m]);
     }
 };
 

@@ -62,16 +62,20 @@ def test_precision_split_vs_fp64_oracle(bec, oracle, nxy, dt, tol_state32, tol_o
 
 def test_fp32_visibility_error_over_long_run(bec):
     """Mean-field run (no noise) at 128x16x16: the FP32 split stays within 1e-4 of the
-    all-FP64 GPU run in visibility over 4000 steps (measured 4e-5).  The total atom
-    number drifts by ~3e-8 per step in FP32 (measured 1.1e-4 after 4000 steps:
-    FP32 unit-modulus factors are not exactly of modulus one, although the
-    in-register twiddles are chosen so that their errors cancel); its bar is 5e-4."""
+    all-FP64 GPU run in visibility over 4000 steps (measured 3.5e-5).  The total atom
+    number drifts in FP32 because unit-modulus factors are not exactly of modulus
+    one, although the twiddles and tables are chosen so that their errors cancel
+    (measured 5.7e-5 after 4000 steps; 1.1e-4 with the twiddles balanced only in
+    pairs and nearest-rounded tables); its bar is 2e-4."""
     from bec_amd import precision_study as P
     r = P.main(["--traj", "8", "--steps", "4000", "--sample-every", "500", "--nxy", "16", "--va", "0", "--vb", "0"])
     e = r["fp32_vs_fp64_gpu"]
+    print("FP32 vs FP64 over 4000 steps: visibility %.3g (grid %.3g), N_total %.3g, phase %.3g"
+          % (e["visibility"]["max_abs"], e["visibility_grid"]["max_abs"], e["N_total"]["max_rel"],
+             e["phase"]["max_abs"]), flush=True)
     assert e["visibility_grid"]["max_abs"] <= 1e-4, e
     assert e["visibility"]["max_abs"] <= 1e-4, e
-    assert e["N_total"]["max_rel"] <= 5e-4, e
+    assert e["N_total"]["max_rel"] <= 2e-4, e
 
 
 def _config1_meanfield_20000(bec):
@@ -85,9 +89,9 @@ def long_run(bec):
     return _config1_meanfield_20000(bec)
 
 
-@pytest.mark.xfail(strict=True, reason="FP32 misses the 1e-4 visibility target over 20000 steps: the "
-                   "transforms' rounding drifts the norm by ~3e-8 per step and the visibility follows "
-                   "the atom number (measured 1.9e-4); the all-FP64 context meets it")
+@pytest.mark.xfail(strict=True, reason="FP32 misses the 1e-4 visibility target over 20000 steps "
+                   "(measured 1.6e-4; the norm drift of the transforms is down to 2.5e-9 per step, and "
+                   "what is left is the rounding of the whole step); the all-FP64 context meets it")
 def test_fp32_visibility_within_1e4_over_20000_steps(long_run):
     """Config-1 physics without noise, 20000 steps (0.8 s): the 1e-4 visibility
     target against the all-FP64 GPU run (which equals the FP64 oracle to 1e-11
@@ -98,13 +102,13 @@ def test_fp32_visibility_within_1e4_over_20000_steps(long_run):
 
 def test_fp32_error_over_20000_steps_stays_at_its_measured_level(long_run):
     """The same run held to what FP32 reaches, so that a regression shows: the
-    visibility within 2.5e-4 (measured 1.9e-4; on the angle grid 1e-3, measured
-    5.8e-4: a drifted phase moves the grid's maximum to the next angle), the
-    atom number within 1e-3 of N (measured 3e-4), the relative phase within
-    5e-3 rad (measured 4.4e-3)."""
+    visibility within 2.5e-4 (measured 1.6e-4; on the angle grid 6e-4, measured
+    4.0e-4: a drifted phase moves the grid's maximum to the next angle), the
+    atom number within 3e-4 of N (measured 1.6e-4), the relative phase within
+    4e-3 rad (measured 2.5e-3)."""
     e = long_run["fp32_vs_fp64_gpu"]
     print("FP32 vs FP64 over 20000 steps:", e, flush=True)
     assert e["visibility"]["max_abs"] <= 2.5e-4, e
-    assert e["visibility_grid"]["max_abs"] <= 1e-3, e
-    assert e["N_total"]["max_rel"] <= 1e-3, e
-    assert e["phase"]["max_abs"] <= 5e-3, e
+    assert e["visibility_grid"]["max_abs"] <= 6e-4, e
+    assert e["N_total"]["max_rel"] <= 3e-4, e
+    assert e["phase"]["max_abs"] <= 4e-3, e
