@@ -419,7 +419,9 @@ int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch);
 /* Candidate schedules for the group's lattice and precision (*count of them). */
 int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *out, int max_out, int *count);
 /* Times every candidate with `steps` alternating z/xy passes on the group's
- * own state (saved and restored) and keeps the fastest; results receives
+ * own state (saved and restored), replayed from a captured graph when the
+ * context uses graphs, four times (the first warms up, the fastest of the
+ * other three counts), and keeps the fastest candidate; results receives
  * per-candidate ms_per_step, *n_results the number of candidates. */
 int bec_autotune(bec_ctx *ctx, int group, int steps, bec_schedule *results, int max_results,
                  int *n_results, bec_schedule *best);
