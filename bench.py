@@ -407,7 +407,8 @@ def main():
     s = B.Solver(c, precision=B.FP32)
     s.set_graphs(not a.no_graphs)
     s.init_evolution(psi, SEED, equilibrate=True)
-    sched = s.autotune(steps=40)[0] if not a.no_autotune else s.schedule()
+    # 200-step trials (5 ms each): the leading schedules here are about 1% apart
+    sched = s.autotune(steps=200)[0] if not a.no_autotune else s.schedule()
     s.prepare(a.steps, sample_every=a.sample_every)   # capture the timed call's step graphs, untimed
     # the warmup runs last, so that the timed call follows work on the GPU rather than host-side capture
     s.propagate(a.warmup, sample_every=a.sample_every)
@@ -426,7 +427,8 @@ def main():
     cb = B.derive(B.default_model(**config1(ne=a.roofline_traj)))
     sb = B.Solver(cb, precision=B.FP32)
     sb.init_evolution(psi, SEED, equilibrate=False)
-    sched_big = sb.autotune(steps=8)[0] if not a.no_autotune else sb.schedule()
+    # 40-step trials (28 ms each here): 8-step ones could not separate schedules 0.7% apart
+    sched_big = sb.autotune(steps=40)[0] if not a.no_autotune else sb.schedule()
     sb.propagate(64)
     ms_big = sb.time_passes(iters=50)
     sb.synchronize()
