@@ -51,7 +51,10 @@ __device__ __forceinline__ void z_group_tile(const MultiZGroup &G, int variant, 
     z_tile<float, NZ, S::R1, S::PW, +1, true, Op, true, true>(op, geo, (const cpx<float> *)G.tw, blk, cb, lds, tid);
 }
 
-__global__ void __launch_bounds__(ZMT) z_multi_kernel(MultiZArgs args, int variant)
+// At most 128 VGPRs (4 waves per SIMD), so that a CU holds two of the 512-thread
+// workgroups; above that it holds one and the pass slows by about a tenth.
+__global__ void __launch_bounds__(ZMT) __attribute__((amdgpu_waves_per_eu(4, 8)))
+z_multi_kernel(MultiZArgs args, int variant)
 {
     __shared__ float lds[Z_LDS_SLOTS];
     for (long t = blockIdx.x; t < args.total_tiles; t += gridDim.x) {
