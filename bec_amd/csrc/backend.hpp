@@ -53,8 +53,10 @@ long xy_workgroups(const DevGroup &g, int ntraj); // partial-sum slots of an xy 
 int xy_wg_per_traj(const DevGroup &g);
 int default_pw(int prec, int nx, int nz);
 int default_r1(int nz);
-// the schedule candidates the hot-pass launchers are instantiated for
-int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out);
+// the schedule candidates the hot-pass launchers are instantiated for; unless
+// `all`, without the forms that cannot win at this batch (the ones the autotuner
+// times and bec_schedule_candidates lists; bec_set_schedule accepts all)
+int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out, bool all = false);
 
 hipError_t run_z_kinetic(const DevGroup &g, int variant, hipStream_t s);
 hipError_t run_xy_nonlinear(const DevGroup &g, hipStream_t s);

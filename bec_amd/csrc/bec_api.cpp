@@ -1334,7 +1334,7 @@ extern "C" int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch
     if (!sch) return BEC_ERR_INVALID;
     Group &g = ctx->groups[group];
     Schedule cand[64];
-    const int n = schedule_candidates(g.dev.prec, g.dev.nx, g.dev.nz, g.dev.ntraj, cand, 64);
+    const int n = schedule_candidates(g.dev.prec, g.dev.nx, g.dev.nz, g.dev.ntraj, cand, 64, true);
     bool ok = false;
     for (int i = 0; i < n && i < 64; ++i)
         if (cand[i].pw == sch->pw && cand[i].z_r1 == sch->z_r1 && cand[i].z_split == sch->z_split &&

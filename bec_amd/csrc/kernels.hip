@@ -26,7 +26,7 @@ int default_pw(int prec, int nx, int nz)
 
 int default_r1(int nz) { return nz == 64 ? 8 : 16; }
 
-int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out)
+int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int max_out, bool all)
 {
     int n = 0;
     auto add = [&](Schedule c) {
@@ -57,12 +57,22 @@ int schedule_candidates(int prec, int nx, int nz, int ntraj, Schedule *out, int 
                 for (int half = 0; half < 2; ++half)
                     for (int xcd = 0; xcd < xcds; ++xcd)
                         if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcd, 1, half});
-    // the split xy pass (one wavefront per component) on the XCD-aware family
-    for (int pw : {32, 64})
-        for (int r1 : r1s)
-            for (int nt = 0; nt < 2; ++nt)
-                for (int half = 0; half < 2; ++half)
-                    if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcds - 1, 1, half, 1});
+    // the split xy pass (one wavefront per component) on the XCD-aware family,
+    // offered only while the two-component pass's wavefronts do not fill the GPU
+    // once (4 per SIMD at 16-point rows, 2 at 32): it exists to double them for
+    // small batches; at 2048 trajectories it is 0.8% slower per step and timed
+    // trials cannot tell the two apart
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;   // MI355X
+    const long xy_waves = (long)ntraj * nz / (64 / nx), xy_slots = (long)cus * 4 * (nx == 16 ? 4 : 2);
+    if (all || xy_waves < xy_slots)
+        for (int pw : {32, 64})
+            for (int r1 : r1s)
+                for (int nt = 0; nt < 2; ++nt)
+                    for (int half = 0; half < 2; ++half)
+                        if (fits(pw, r1, half)) add(Schedule{pw, r1, 1, nt, xcds - 1, 1, half, 1});
     // the xy occupancy target on the streaming, XCD-aware family
     for (int pw : {32, 64})
         for (int half = 0; half < 2; ++half)

@@ -416,7 +416,10 @@ typedef struct bec_schedule {
 
 int bec_set_schedule(bec_ctx *ctx, int group, const bec_schedule *sch);
 int bec_get_schedule(bec_ctx *ctx, int group, bec_schedule *sch);
-/* Candidate schedules for the group's lattice and precision (*count of them). */
+/* Candidate schedules for the group's lattice, precision and batch (*count of
+ * them): the ones bec_autotune times.  The split xy pass is among them only
+ * while the two-component pass's wavefronts do not fill the GPU once;
+ * bec_set_schedule accepts it at any batch. */
 int bec_schedule_candidates(bec_ctx *ctx, int group, bec_schedule *out, int max_out, int *count);
 /* Times every candidate with `steps` alternating z/xy passes on the group's
  * own state (saved and restored), replayed from a captured graph when the
